@@ -1,0 +1,180 @@
+"""Benchmark: Barnes-Hut leapfrog steps on one MI355X.
+
+Workload (default): two-disk encounter, 1,048,576 particles, FP32 state,
+Barnes-Hut LBVH rebuilt every step with 30-bit Morton keys, theta = 0.5,
+eps = 0.02, dt = 0.005, wave64 group walk.  A step = fused kick+drift+bbox,
+Morton, onesweep sort, Karras build, COM, tree walk + closing kick; inputs are
+resident in HBM when the timed region starts.
+
+Prints one JSON line (the metric is particle-steps/s) with:
+  roofline      the tree walk kernel (the dominant one) against the FP32 VALU
+                peak: algorithmic flops = 20 per interaction + 10 per opening
+                test of the per-target walk on the same tree (counted on the
+                device), divided by the walk kernel's mean duration measured
+                with HIP events on the engine stream during the timed steps;
+  accuracy      median / p99 relative force error of the final state's
+                accelerations against an FP64 all-pairs oracle on sampled
+                targets;
+  cpu_baseline  the reference CPU algorithm (NumPy chunked float32 all-pairs
+                + KDK, scripts/reference_nbody.py:45-61,130-136 restated)
+                timed on this host on a bounded sample and extrapolated.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector peak
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(n_target, sample_n, eps, dt):
+    """reference_nbody.py's algorithm (NumPy, float32, chunk 2048) on one step
+    of a sample_n-particle two-disk system; extrapolated to n_target as
+    O(n^2) all-pairs (the reference's CPU path has no tree code at scale)."""
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    from nbodyssey_amd import ic
+    from oracle import numpy_ref
+    pos4, vel4 = ic.two_disk_encounter(sample_n, seed=1, eps=eps)
+    pos, mass, vel = pos4[:, :3].copy(), pos4[:, 3].copy(), vel4[:, :3].copy()
+    t0 = time.perf_counter()
+    numpy_ref.kdk_run(pos, vel, mass, eps, dt, 1)
+    sec = time.perf_counter() - t0   # a(x0) + one step = 2 force evaluations
+    per_step = sec / 2.0 * (n_target / sample_n) ** 2
+    return {"value": n_target / per_step, "unit": "particle-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": f"NumPy float32 chunked all-pairs KDK (reference_nbody.py algorithm), "
+                      f"{sample_n} particles, 2 force evaluations in {sec:.1f} s on 1 of "
+                      f"{os.cpu_count()} host cores ({cpu_model()}), extrapolated x"
+                      f"{(n_target / sample_n) ** 2:.0f} to {n_target} particles (O(n^2))"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--theta", type=float, default=0.5)
+    ap.add_argument("--traverse", default="warp", choices=["warp", "thread"])
+    ap.add_argument("--group-width", type=int, default=64)
+    ap.add_argument("--key-bits", type=int, default=30)
+    ap.add_argument("--cpu-sample", type=int, default=16384)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--accuracy-targets", type=int, default=256)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the hot path (one N-body system per GPU) has "
+              "no multi-GPU mode in this build; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from nbodyssey_amd import engine, ic
+    eps, dt = 0.02, 0.005
+    t_ic = time.perf_counter()
+    pos, vel = ic.two_disk_encounter(args.n, seed=1, eps=eps)
+    t_ic = time.perf_counter() - t_ic
+    print(f"bench: n={args.n} ic {t_ic:.1f}s", file=sys.stderr, flush=True)
+
+    sim = engine.Simulation(pos, vel, dt=dt, eps=eps, theta=args.theta, force="bh",
+                            traverse=args.traverse, group_width=args.group_width,
+                            key_bits=args.key_bits)
+    sim.compute_forces()              # a(x0)
+    sim.step(args.warmup)
+    sim.synchronize()
+    print("bench: warmup done", file=sys.stderr, flush=True)
+
+    sim.reset_force_timing()
+    sim.synchronize()
+    t0 = time.perf_counter()
+    ms_events = sim.time_steps(args.steps)
+    sim.synchronize()
+    wall = time.perf_counter() - t0
+    phase, calls = sim.bh_phase_ms()
+    ms_per_step = ms_events / args.steps
+    value = args.n * args.steps / (ms_events * 1e-3)
+    print(f"bench: {args.steps} steps {ms_events:.2f} ms (wall {wall * 1e3:.2f} ms)",
+          file=sys.stderr, flush=True)
+
+    # algorithmic flops of the walk on the final tree: per-target walk counts
+    inter, opens = sim.traverse_counts("thread")
+    flops = 20.0 * inter.astype(np.float64).sum() + 10.0 * opens.astype(np.float64).sum()
+    walk_ms = phase[5] / max(calls, 1)
+    achieved = flops / (walk_ms * 1e-3) / 1e12
+    ginter, gopen = sim.traverse_counts(args.traverse, args.group_width)
+    width = 1 if args.traverse == "thread" else args.group_width
+    lane_inter = ginter.astype(np.float64).sum() * width if width > 1 else ginter.sum()
+
+    # accuracy of the final state's forces vs FP64 all-pairs on sampled targets
+    acc = sim.acc()
+    p_final, _ = sim.download()
+    accuracy = None
+    try:
+        from oracle import orc
+        idx = np.random.default_rng(0).choice(args.n, args.accuracy_targets,
+                                              replace=False).astype(np.int32)
+        ref = orc.allpairs_f64(p_final, eps, targets=idx)
+        err = np.linalg.norm(acc[idx, :3] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        accuracy = {"median_rel_force_err_vs_fp64": float(np.median(err)),
+                    "p99_rel_force_err_vs_fp64": float(np.quantile(err, 0.99)),
+                    "targets": int(args.accuracy_targets)}
+    except Exception as exc:  # the oracle is a checker; report, do not fail the bench
+        accuracy = {"error": str(exc)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args.n, args.cpu_sample, eps, dt)
+
+    names = ["bbox", "morton", "sort", "build", "com", "traverse"]
+    out = {
+        "metric": "particle-steps/s @1M BH θ=0.5 incl. rebuild",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic two-disk encounter (seeded generator)",
+        "config": {"workload": f"two-disk {args.n} particles, Barnes-Hut LBVH rebuilt every step, "
+                               f"{args.key_bits}-bit keys, theta={args.theta}, eps={eps}, dt={dt}, "
+                               f"walk={args.traverse}{'' if width == 1 else width}",
+                   "n": args.n, "theta": args.theta, "key_bits": args.key_bits,
+                   "traverse": args.traverse, "group_width": width},
+        "phase_ms": {k: phase[i] / max(calls, 1) for i, k in enumerate(names)},
+        "roofline": {"bound": "valu", "kernel": "walk",
+                     "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "flops_per_launch": flops, "walk_ms": walk_ms,
+                     "per_target_interactions": float(inter.mean()),
+                     "per_target_opens": float(opens.mean()),
+                     "group_lane_interactions_per_target": float(lane_inter / args.n)},
+        "accuracy": accuracy,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    sim.close()
+
+
+if __name__ == "__main__":
+    main()

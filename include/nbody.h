@@ -1,0 +1,158 @@
+/*
+ * nbody.h -- C ABI of libnbody.so, the MI355X (gfx950) N-body engine.
+ *
+ * Plain C types only (pointers, sizes, scalars).  Every function returns an
+ * int status (NB_OK = 0) unless stated otherwise; nb_last_error() gives the
+ * message of the last failure on the calling thread.  The reference aborts
+ * the process instead (src/cuda_check.cuh:7-15); the galaxy_sim driver turns
+ * a nonzero status back into the reference's message and exit(1).
+ *
+ * Host arrays use the reference's float4 layout (docs/FORMATS.md:11-16):
+ * pos4 = n x (x, y, z, m), vel4 = n x (vx, vy, vz, 0), acc4 = n x (ax, ay, az, 0),
+ * all in original particle order.  A handle owns its device state and one HIP
+ * stream; one thread may use a handle at a time.
+ *
+ * The reference binds this path through a C++ API (src/forces.cuh,
+ * src/integrator.cuh, src/simulation.hpp); the same-named C++ functions are
+ * declared in include/nbody/simulation.hpp, forces.hpp and integrator.hpp.
+ * Each entry point below names the reference interface it replaces.
+ */
+#ifndef NBODY_H
+#define NBODY_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NB_OK 0
+#define NB_ERR_INVALID 1   /* bad argument */
+#define NB_ERR_DEVICE 2    /* HIP runtime error / no device */
+#define NB_ERR_NOMEM 3     /* device allocation failed */
+#define NB_ERR_STATE 4     /* call made out of order */
+#define NB_ERR_OVERFLOW 5  /* a tree-walk stack would have overflowed */
+
+/* ForceMethod, src/simulation.hpp:4 */
+#define NB_FORCE_NAIVE 0
+#define NB_FORCE_BH 1
+/* BhTraversal, src/simulation.hpp:5.  NB_TRAVERSE_WARP is the group-vote walk
+ * of traverse_warp_kernel (src/forces_bh.cu:395-447) over `group_width` lanes
+ * (16, 32 = the reference's warp, or 64 = one wave64). */
+#define NB_TRAVERSE_THREAD 0
+#define NB_TRAVERSE_WARP 1
+/* accumulation of per-particle sums across tiles (all-pairs) */
+#define NB_PREC_FP32 0
+#define NB_PREC_COMPENSATED 1
+
+/* SimParams, src/simulation.hpp:8-15, plus the additive knobs. */
+typedef struct nb_params {
+  float dt;        /* timestep */
+  float epsilon;   /* softening length */
+  float G;         /* gravitational constant */
+  float theta;     /* Barnes-Hut opening angle; 0 = exact */
+  int force;       /* NB_FORCE_* */
+  int traverse;    /* NB_TRAVERSE_* */
+  int key_bits;    /* Morton key width: 63 (reference) or 30 */
+  int group_width; /* lanes per group for NB_TRAVERSE_WARP: 16, 32 or 64 */
+  int precision;   /* NB_PREC_* */
+} nb_params;
+
+typedef struct nb_engine nb_engine;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* nb_last_error(void);
+/* Library version string. */
+const char* nb_version(void);
+/* Number of visible HIP devices (0 when there is none); never fails. */
+int nb_device_count(void);
+
+/* Reference defaults: main.cu:19-25 and simulation.hpp:11-14 (dt 0.01,
+ * eps 0.05, G 1, theta 0.5, naive, thread walk), key_bits 63, width 32. */
+void nb_default_params(nb_params* p);
+
+/* ParticleSystem::allocate (src/simulation.cu:4-8): device state for n
+ * particles (n >= 1) on the current device. */
+int nb_create(int n, nb_engine** out);
+/* ParticleSystem::release + bh_release (src/simulation.cu:10-15,
+ * src/forces_bh.cu:541-548). */
+int nb_destroy(nb_engine* h);
+int nb_set_params(nb_engine* h, const nb_params* p);
+int nb_get_params(const nb_engine* h, nb_params* p);
+int nb_count(const nb_engine* h);
+
+/* ParticleSystem::upload (src/simulation.cu:17-20).  vel4 may be NULL (zero). */
+int nb_upload(nb_engine* h, const float* pos4, const float* vel4);
+/* ParticleSystem::download_positions (src/simulation.cu:22-24); vel4 may be
+ * NULL.  Waits for queued work. */
+int nb_download(nb_engine* h, float* pos4, float* vel4);
+/* Copies the engine's acceleration array (a(x) of the last force evaluation). */
+int nb_download_acc(nb_engine* h, float* acc4);
+
+/* compute_forces (src/forces.cuh:8, src/forces.cu:68-86): evaluates the force
+ * module chosen by the params on the current positions into the engine's
+ * acceleration array, then copies it to acc4 if acc4 is not NULL.  Timed into
+ * nb_force_ms_total like the reference dispatcher. */
+int nb_compute_forces(nb_engine* h, float* acc4);
+
+/* leapfrog_step x nsteps (src/integrator.cuh:12-19, src/integrator.cu:27-37).
+ * If no force evaluation has happened since the last upload, a(x0) is
+ * computed first (main.cu:184).  Enqueued asynchronously; nb_synchronize or
+ * any download waits. */
+int nb_step(nb_engine* h, int nsteps);
+int nb_synchronize(nb_engine* h);
+
+/* total_energy (src/integrator.cuh:21, src/integrator.cu:93-110):
+ * out = {total, kinetic, potential} reduced in FP64. */
+int nb_total_energy(nb_engine* h, double out[3]);
+
+/* force_kernel_ms_total / force_kernel_calls / reset_force_timing
+ * (src/forces.cuh:17-19).  Wait for queued work. */
+double nb_force_ms_total(nb_engine* h);
+long nb_force_calls(nb_engine* h);
+void nb_reset_force_timing(nb_engine* h);
+/* Per-call phase timing off/on (default on).  Off removes the event records
+ * between phases; the force-call total is still kept. */
+int nb_set_phase_timing(nb_engine* h, int on);
+/* bh_report_phase_timing data (src/forces_bh.cu:529-539): accumulated ms of
+ * the six phases bbox, morton, sort, build, com, traverse and the call count. */
+int nb_bh_phase_ms(nb_engine* h, double out[6], long* calls);
+/* bh_release (src/forces_bh.cu:541-548): frees the tree scratch. */
+int nb_bh_release(nb_engine* h);
+
+/* Runs nsteps leapfrog steps bracketed by two HIP events on the engine stream
+ * and returns the elapsed milliseconds (no host synchronisation inside). */
+int nb_time_steps(nb_engine* h, int nsteps, double* ms);
+
+/* ---- stage-level entry points (the reference keeps these stages in an
+ * anonymous namespace, src/forces_bh.cu:39-461); used by the bit-exact
+ * parity checks. */
+
+/* Tree of the last Barnes-Hut evaluation: keys in sorted order (widened to
+ * 64 bits), perm (original index of each sorted slot), children (2*(n-1)),
+ * parent (2n-1), node COM (4*(n-1)), node size^2 (n-1), positions in sorted
+ * order (4*n).  Any pointer may be NULL. */
+int nb_get_tree(nb_engine* h, uint64_t* keys_sorted, uint32_t* perm, int32_t* children,
+                int32_t* parent, float* com4, float* size2, float* pos_sorted4);
+/* Runs the given walk on the current tree and returns its interaction and
+ * opening-test counts: per sorted target for NB_TRAVERSE_THREAD, per group of
+ * group_width consecutive sorted targets for NB_TRAVERSE_WARP.  Does not
+ * touch the acceleration array. */
+int nb_traverse_counts(nb_engine* h, int traverse, int group_width, int32_t* interactions,
+                       int32_t* opens);
+/* morton_kernel (src/forces_bh.cu:192-220) incl. the bbox reduction: keys of
+ * n positions, key_bits 63 or 30. */
+int nb_morton(const float* pos4, int n, int key_bits, uint64_t* keys);
+/* cub::DeviceRadixSort::SortPairs (src/forces_bh.cu:490-497) with vals = index:
+ * stable sort of keys (key_bits 30: keys < 2^30; 63: any 64-bit key). */
+int nb_sort_pairs(const uint64_t* keys, int n, int key_bits, uint64_t* keys_sorted,
+                  uint32_t* perm);
+/* build_kernel (src/forces_bh.cu:242-277) on sorted keys. */
+int nb_karras_build(const uint64_t* keys_sorted, int n, int key_bits, int32_t* children,
+                    int32_t* parent);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NBODY_H */

@@ -1,0 +1,129 @@
+"""ctypes binding of libnbody.so (declarations in include/nbody.h).
+
+The library is loaded from this package directory only; if it is missing the
+import fails loudly -- there is no CPU fallback for the product path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnbody.so")
+
+NB_OK = 0
+NB_ERR_INVALID = 1
+NB_ERR_DEVICE = 2
+NB_ERR_NOMEM = 3
+NB_ERR_STATE = 4
+NB_ERR_OVERFLOW = 5
+
+FORCE_NAIVE = 0
+FORCE_BH = 1
+TRAVERSE_THREAD = 0
+TRAVERSE_WARP = 1
+PREC_FP32 = 0
+PREC_COMPENSATED = 1
+
+
+class NbParams(ctypes.Structure):
+    _fields_ = [
+        ("dt", ctypes.c_float),
+        ("epsilon", ctypes.c_float),
+        ("G", ctypes.c_float),
+        ("theta", ctypes.c_float),
+        ("force", ctypes.c_int),
+        ("traverse", ctypes.c_int),
+        ("key_bits", ctypes.c_int),
+        ("group_width", ctypes.c_int),
+        ("precision", ctypes.c_int),
+    ]
+
+
+class NbError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"libnbody error {code}: {msg}")
+        self.code = code
+
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_cp = ctypes.c_char_p
+_dp = ctypes.POINTER(ctypes.c_double)
+_lp = ctypes.POINTER(ctypes.c_long)
+_pp = ctypes.POINTER(NbParams)
+
+# name -> (restype, argtypes); every function declared in include/nbody.h
+SIGNATURES = {
+    "nb_last_error": (_cp, []),
+    "nb_version": (_cp, []),
+    "nb_device_count": (_i, []),
+    "nb_default_params": (None, [_pp]),
+    "nb_create": (_i, [_i, ctypes.POINTER(_vp)]),
+    "nb_destroy": (_i, [_vp]),
+    "nb_set_params": (_i, [_vp, _pp]),
+    "nb_get_params": (_i, [_vp, _pp]),
+    "nb_count": (_i, [_vp]),
+    "nb_upload": (_i, [_vp, _vp, _vp]),
+    "nb_download": (_i, [_vp, _vp, _vp]),
+    "nb_download_acc": (_i, [_vp, _vp]),
+    "nb_compute_forces": (_i, [_vp, _vp]),
+    "nb_step": (_i, [_vp, _i]),
+    "nb_synchronize": (_i, [_vp]),
+    "nb_total_energy": (_i, [_vp, _vp]),
+    "nb_force_ms_total": (ctypes.c_double, [_vp]),
+    "nb_force_calls": (ctypes.c_long, [_vp]),
+    "nb_reset_force_timing": (None, [_vp]),
+    "nb_set_phase_timing": (_i, [_vp, _i]),
+    "nb_bh_phase_ms": (_i, [_vp, _vp, _lp]),
+    "nb_bh_release": (_i, [_vp]),
+    "nb_time_steps": (_i, [_vp, _i, _dp]),
+    "nb_get_tree": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "nb_traverse_counts": (_i, [_vp, _i, _i, _vp, _vp]),
+    "nb_morton": (_i, [_vp, _i, _i, _vp]),
+    "nb_sort_pairs": (_i, [_vp, _i, _i, _vp, _vp]),
+    "nb_karras_build": (_i, [_vp, _i, _i, _vp, _vp]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libnbody.so from the package directory (raises if it is absent)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
+                f"g.build()'` or `make -C nbodyssey_amd/csrc` (no CPU fallback exists)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status):
+    if status != NB_OK:
+        raise NbError(status, load().nb_last_error().decode(errors="replace"))
+    return status
+
+
+def ptr(a):
+    """Raw pointer of a C-contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
+    return a.ctypes.data
+
+
+def as_f4(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.ndim != 2 or a.shape[1] != 4:
+        a = a.reshape(-1, 4)
+    if n is not None and len(a) != n:
+        raise ValueError(f"expected {n} float4 rows, got {len(a)}")
+    return a

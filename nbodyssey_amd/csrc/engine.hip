@@ -1,0 +1,957 @@
+// Host side of libnbody.so: the engine (device buffers, stream, timing), the
+// C ABI of include/nbody.h and the reference-named C++ API of
+// include/nbody/{simulation,forces,integrator}.hpp.
+//
+// Step structure (one leapfrog step, integrator.cu:27-37, all on one stream,
+// no host synchronisation):
+//   naive: kick_drift -> all-pairs (+ closing kick in its epilogue)
+//   BH:    kick_drift_bbox (last block writes the FP64 quantisation and resets
+//          the sort state) -> morton+histograms -> onesweep passes (the last
+//          one gathers pos_sorted) -> Karras -> COM -> walk (+ closing kick)
+#include "nb_internal.hpp"
+#include "nbody.h"
+#include "nbody/forces.hpp"
+#include "nbody/integrator.hpp"
+#include "nbody/simulation.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+namespace nb {
+namespace {
+
+thread_local std::string g_last_error;
+
+enum Phase { kPhBbox, kPhMorton, kPhSort, kPhBuild, kPhCom, kPhWalk, kPhases };
+const char* kPhaseNames[kPhases] = {"bbox", "morton", "sort", "build", "com", "traverse"};
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t count = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { reset(); }
+  void alloc(size_t c, bool zero = false) {
+    reset();
+    if (c == 0) c = 1;
+    NB_HIP(hipMalloc(reinterpret_cast<void**>(&p), c * sizeof(T)));
+    count = c;
+    if (zero) NB_HIP(hipMemset(p, 0, c * sizeof(T)));
+  }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    count = 0;
+  }
+};
+
+void check_device() {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+    throw Error(kErrDevice, "no HIP device is visible (libnbody needs an MI355X / gfx950 GPU)");
+}
+
+struct Params {
+  float dt = 0.01f, eps = 0.05f, G = 1.0f, theta = 0.5f;
+  int force = 0, traverse = 0, key_bits = 63, group_width = 32, precision = 0;
+};
+
+void validate(const Params& p) {
+  if (p.force != 0 && p.force != 1) throw Error(kErrInvalid, "force must be 0 (naive) or 1 (bh)");
+  if (p.traverse != 0 && p.traverse != 1)
+    throw Error(kErrInvalid, "traverse must be 0 (thread) or 1 (warp)");
+  if (p.key_bits != 30 && p.key_bits != 63) throw Error(kErrInvalid, "key_bits must be 30 or 63");
+  if (p.group_width != 16 && p.group_width != 32 && p.group_width != 64)
+    throw Error(kErrInvalid, "group_width must be 16, 32 or 64");
+  if (p.precision != 0 && p.precision != 1) throw Error(kErrInvalid, "precision must be 0 or 1");
+  if (!(p.theta >= 0.f)) throw Error(kErrInvalid, "theta must be >= 0");
+  if (!(p.eps >= 0.f)) throw Error(kErrInvalid, "epsilon must be >= 0");
+}
+
+// Events of one timed force call, resolved lazily (no host sync per call).
+struct CallEvents {
+  hipEvent_t ev[kPhases + 1];
+  int count;  // 2 (total only) or kPhases + 1
+  bool bh;
+};
+
+class Engine {
+ public:
+  explicit Engine(int n_) : n(n_) {
+    check_device();
+    if (n < 1) throw Error(kErrInvalid, "particle count must be >= 1");
+    NB_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    own_pos.alloc((size_t)n, true);
+    own_vel.alloc((size_t)n, true);
+    own_acc.alloc((size_t)n, true);
+    pos = own_pos.p;
+    vel = own_vel.p;
+    acc = own_acc.p;
+  }
+  // Non-owning engine over caller device arrays, on the null stream (the
+  // reference's default-stream semantics).
+  Engine() : n(0) { check_device(); }
+  ~Engine() {
+    for (auto& c : pending_) release_events(c);
+    for (auto e : free_events_) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void bind(float4* p, float4* v, float4* a, int count) {
+    if (count != n) acc_valid = false;
+    pos = p;
+    vel = v;
+    acc = a;
+    n = count;
+  }
+
+  void set_params(const Params& np) {
+    validate(np);
+    if (np.key_bits != p.key_bits) release_bh();
+    p = np;
+  }
+
+  // ---------------------------------------------------------------- forces
+  // Enqueue one force evaluation on the current positions.  kick: this is
+  // the force half of a leapfrog step, so the first kick + drift run before
+  // it (fused with the bbox for BH) and the closing kick after it.
+  void force_pass(bool step) {
+    const float half = 0.5f * p.dt;
+    const float eps2 = p.eps * p.eps;
+    const bool bh = p.force == 1;
+    CallEvents ce{};
+    const bool timed = timing_on;
+    if (timed) begin_call(ce, bh && phase_timing);
+    if (bh && n >= 2) {
+      ensure_bh();
+      const int passes = sort_passes(p.key_bits);
+      if (step)
+        launch_kick_drift(stream, pos, vel, acc, n, half, p.dt, true, bh_partial.p, state.p,
+                          quant.p, passes);
+      else
+        launch_bbox(stream, pos, n, bh_partial.p, state.p, quant.p, passes);
+      mark(ce, kPhBbox + 1);
+      launch_morton_hist(stream, pos, n, quant.p, p.key_bits, keys_a.p, state.p);
+      mark(ce, kPhMorton + 1);
+      launch_sort(stream, p.key_bits, keys_a.p, keys_b.p, vals_a.p, vals_b.p, n, state.p,
+                  lookback.p, max_tiles, true, pos, pos_sorted.p);
+      mark(ce, kPhSort + 1);
+      launch_karras(stream, p.key_bits, keys_a.p, n, nodes.p, parent.p, visit.p);
+      mark(ce, kPhBuild + 1);
+      launch_com(stream, pos_sorted.p, n, nodes.p, parent.p, visit.p, box_lo.p, box_hi.p,
+                 &state.p->error);
+      mark(ce, kPhCom + 1);
+      WalkArgs w{};
+      w.pos_sorted = pos_sorted.p;
+      w.perm = vals_a.p;
+      w.n = n;
+      w.nodes = nodes.p;
+      w.theta2 = p.theta * p.theta;
+      w.eps2 = eps2;
+      w.G = p.G;
+      w.acc = acc;
+      w.vel = step ? vel : nullptr;
+      w.half_dt = half;
+      w.overflow = &state.p->overflow;
+      w.error = &state.p->error;
+      launch_walk(stream, p.traverse, p.group_width, w);
+      mark(ce, kPhWalk + 1);
+      tree_valid = true;
+    } else if (bh) {
+      // n < 2: zero force (forces_bh.cu:466-469)
+      if (step) launch_kick_drift(stream, pos, vel, acc, n, half, p.dt, false, nullptr, nullptr,
+                                  nullptr, 0);
+      launch_fill_u32(stream, reinterpret_cast<unsigned*>(acc), 0u, (size_t)n * 4);
+      for (int k = 1; k <= kPhases; ++k) mark(ce, k);
+    } else {
+      if (step) launch_kick_drift(stream, pos, vel, acc, n, half, p.dt, false, nullptr, nullptr,
+                                  nullptr, 0);
+      int split = allpairs_split(n);
+      if (split > 1 && ap_partial.count < (size_t)split * n) ap_partial.alloc((size_t)split * n);
+      launch_allpairs(stream, pos, n, eps2, p.G, acc, ap_partial.p, split, p.precision == 1,
+                      step ? vel : nullptr, half);
+      mark(ce, 1);
+    }
+    if (timed) end_call(ce);
+    acc_valid = true;
+  }
+
+  void step(int k) {
+    if (k < 0) throw Error(kErrInvalid, "nsteps must be >= 0");
+    if (k > 0 && !acc_valid) force_pass(false);
+    for (int s = 0; s < k; ++s) force_pass(true);
+  }
+
+  void check_overflow() {
+    if (!state.p) return;
+    int flag[2] = {0, 0};
+    NB_HIP(hipMemcpyAsync(flag, &state.p->overflow, 2 * sizeof(int), hipMemcpyDeviceToHost,
+                          stream));
+    NB_HIP(hipStreamSynchronize(stream));
+    if (flag[0] || flag[1]) {
+      int zero[2] = {0, 0};
+      NB_HIP(hipMemcpy(&state.p->overflow, zero, 2 * sizeof(int), hipMemcpyHostToDevice));
+      if (flag[0]) throw Error(kErrOverflow, "tree walk stack overflow (depth > 128)");
+      throw Error(kErrState, "inconsistent sort/tree data detected on the device");
+    }
+  }
+
+  void sync() {
+    NB_HIP(hipStreamSynchronize(stream));
+    resolve();
+    check_overflow();
+  }
+
+  void energy(double out[3]) {
+    if (n < 1) throw Error(kErrState, "no particles");
+    int np = energy_partials(n);
+    if ((int)en_partial.count < 2 * np) en_partial.alloc((size_t)2 * np);
+    if (en_out.count < 3) en_out.alloc(3);
+    launch_energy(stream, pos, vel, n, p.eps * p.eps, p.G, en_partial.p, np, en_out.p);
+    NB_HIP(hipMemcpyAsync(out, en_out.p, 3 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    NB_HIP(hipStreamSynchronize(stream));
+  }
+
+  // ---------------------------------------------------------------- scratch
+  void ensure_bh() {
+    if (bh_cap >= n) return;
+    release_bh();
+    const size_t c = (size_t)n;
+    keys_a.alloc(c);
+    keys_b.alloc(c);
+    vals_a.alloc(c);
+    vals_b.alloc(c);
+    pos_sorted.alloc(c);
+    nodes.alloc(c);
+    parent.alloc(2 * c);
+    visit.alloc(c);
+    box_lo.alloc(c);
+    box_hi.alloc(c);
+    bh_partial.alloc(2 * kBboxBlocks);
+    quant.alloc(1, true);
+    state.alloc(1, true);
+    max_tiles = (n + kSortTile - 1) / kSortTile;
+    lookback.alloc((size_t)kMaxPasses * max_tiles * 256, true);
+    bh_cap = n;
+  }
+  void release_bh() {
+    keys_a.reset(); keys_b.reset(); vals_a.reset(); vals_b.reset(); pos_sorted.reset();
+    nodes.reset(); parent.reset(); visit.reset(); box_lo.reset(); box_hi.reset();
+    bh_partial.reset(); quant.reset(); state.reset(); lookback.reset();
+    bh_cap = 0;
+    tree_valid = false;
+  }
+
+  // ---------------------------------------------------------------- timing
+  hipEvent_t get_event() {
+    if (!free_events_.empty()) {
+      hipEvent_t e = free_events_.back();
+      free_events_.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    NB_HIP(hipEventCreate(&e));
+    return e;
+  }
+  void release_events(CallEvents& c) {
+    for (int k = 0; k < c.count; ++k) free_events_.push_back(c.ev[k]);
+    c.count = 0;
+  }
+  void begin_call(CallEvents& c, bool phases) {
+    c.bh = phases;
+    c.count = phases ? kPhases + 1 : 2;
+    for (int k = 0; k < c.count; ++k) c.ev[k] = get_event();
+    NB_HIP(hipEventRecord(c.ev[0], stream));
+  }
+  void mark(CallEvents& c, int k) {
+    if (!timing_on || !c.bh || k >= c.count - 1) return;
+    NB_HIP(hipEventRecord(c.ev[k], stream));
+  }
+  void end_call(CallEvents& c) {
+    NB_HIP(hipEventRecord(c.ev[c.count - 1], stream));
+    pending_.push_back(c);
+    if (pending_.size() > 512) resolve();
+  }
+  void resolve() {
+    if (pending_.empty()) return;
+    NB_HIP(hipEventSynchronize(pending_.back().ev[pending_.back().count - 1]));
+    for (auto& c : pending_) {
+      float ms = 0.f;
+      NB_HIP(hipEventElapsedTime(&ms, c.ev[0], c.ev[c.count - 1]));
+      force_ms += ms;
+      ++force_calls;
+      if (c.bh) {
+        for (int k = 0; k < kPhases; ++k) {
+          float pm = 0.f;
+          NB_HIP(hipEventElapsedTime(&pm, c.ev[k], c.ev[k + 1]));
+          phase_ms[k] += pm;
+        }
+        ++bh_calls;
+      }
+      release_events(c);
+    }
+    pending_.clear();
+  }
+  void reset_timing() {
+    resolve();
+    force_ms = 0.0;
+    force_calls = 0;
+    bh_calls = 0;
+    for (double& x : phase_ms) x = 0.0;
+  }
+
+  int n;
+  float4* pos = nullptr;
+  float4* vel = nullptr;
+  float4* acc = nullptr;
+  hipStream_t stream = nullptr;
+  Params p;
+  bool acc_valid = false;
+  bool tree_valid = false;
+  bool timing_on = true;
+  bool phase_timing = true;
+  double force_ms = 0.0;
+  long force_calls = 0;
+  double phase_ms[kPhases] = {};
+  long bh_calls = 0;
+
+  DevBuf<float4> own_pos, own_vel, own_acc;
+  DevBuf<float4> ap_partial;
+  DevBuf<double> en_partial, en_out;
+  int bh_cap = 0, max_tiles = 0;
+  DevBuf<uint64_t> keys_a, keys_b;
+  DevBuf<unsigned> vals_a, vals_b, visit;
+  DevBuf<float4> pos_sorted, box_lo, box_hi, bh_partial;
+  DevBuf<Node> nodes;
+  DevBuf<int> parent;
+  DevBuf<QuantParams> quant;
+  DevBuf<SortState> state;
+  DevBuf<unsigned long long> lookback;
+
+ private:
+  std::vector<hipEvent_t> free_events_;
+  std::vector<CallEvents> pending_;
+};
+
+Params from_c(const nb_params* c) {
+  Params p;
+  p.dt = c->dt;
+  p.eps = c->epsilon;
+  p.G = c->G;
+  p.theta = c->theta;
+  p.force = c->force;
+  p.traverse = c->traverse;
+  p.key_bits = c->key_bits;
+  p.group_width = c->group_width;
+  p.precision = c->precision;
+  return p;
+}
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    return kOk;
+  } catch (const Error& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return kErrNoMem;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return kErrDevice;
+  }
+}
+
+void need(const void* ptr, const char* what) {
+  if (!ptr) throw Error(kErrInvalid, std::string(what) + " is NULL");
+}
+
+// Host-buffer helpers for the stage-level entry points.
+template <typename T>
+void upload(DevBuf<T>& b, const T* h, size_t count) {
+  b.alloc(count);
+  NB_HIP(hipMemcpy(b.p, h, count * sizeof(T), hipMemcpyHostToDevice));
+}
+
+}  // namespace
+}  // namespace nb
+
+struct nb_engine {
+  std::unique_ptr<nb::Engine> e;
+};
+
+using nb::guarded;
+using nb::need;
+
+extern "C" {
+
+const char* nb_last_error(void) { return nb::g_last_error.c_str(); }
+const char* nb_version(void) { return "nbody-mi355x 0.1 (gfx950)"; }
+
+int nb_device_count(void) {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+  return c;
+}
+
+void nb_default_params(nb_params* p) {
+  if (!p) return;
+  p->dt = 0.01f;
+  p->epsilon = 0.05f;
+  p->G = 1.0f;
+  p->theta = 0.5f;
+  p->force = NB_FORCE_NAIVE;
+  p->traverse = NB_TRAVERSE_THREAD;
+  p->key_bits = 63;
+  p->group_width = 32;
+  p->precision = NB_PREC_FP32;
+}
+
+int nb_create(int n, nb_engine** out) {
+  return guarded([&] {
+    need(out, "out");
+    *out = nullptr;
+    auto h = std::make_unique<nb_engine>();
+    h->e = std::make_unique<nb::Engine>(n);
+    *out = h.release();
+  });
+}
+
+int nb_destroy(nb_engine* h) {
+  return guarded([&] {
+    if (!h) return;
+    if (h->e && h->e->stream) (void)hipStreamSynchronize(h->e->stream);
+    delete h;
+  });
+}
+
+int nb_set_params(nb_engine* h, const nb_params* p) {
+  return guarded([&] {
+    need(h, "engine");
+    need(p, "params");
+    h->e->set_params(nb::from_c(p));
+  });
+}
+
+int nb_get_params(const nb_engine* h, nb_params* out) {
+  return guarded([&] {
+    need(h, "engine");
+    need(out, "params");
+    const nb::Params& p = h->e->p;
+    out->dt = p.dt;
+    out->epsilon = p.eps;
+    out->G = p.G;
+    out->theta = p.theta;
+    out->force = p.force;
+    out->traverse = p.traverse;
+    out->key_bits = p.key_bits;
+    out->group_width = p.group_width;
+    out->precision = p.precision;
+  });
+}
+
+int nb_count(const nb_engine* h) { return h ? h->e->n : -1; }
+
+int nb_upload(nb_engine* h, const float* pos4, const float* vel4) {
+  return guarded([&] {
+    need(h, "engine");
+    need(pos4, "pos4");
+    nb::Engine& e = *h->e;
+    size_t bytes = (size_t)e.n * sizeof(float4);
+    NB_HIP(hipMemcpyAsync(e.pos, pos4, bytes, hipMemcpyHostToDevice, e.stream));
+    if (vel4)
+      NB_HIP(hipMemcpyAsync(e.vel, vel4, bytes, hipMemcpyHostToDevice, e.stream));
+    else
+      NB_HIP(hipMemsetAsync(e.vel, 0, bytes, e.stream));
+    NB_HIP(hipStreamSynchronize(e.stream));
+    e.acc_valid = false;
+    e.tree_valid = false;
+  });
+}
+
+int nb_download(nb_engine* h, float* pos4, float* vel4) {
+  return guarded([&] {
+    need(h, "engine");
+    nb::Engine& e = *h->e;
+    size_t bytes = (size_t)e.n * sizeof(float4);
+    if (pos4) NB_HIP(hipMemcpyAsync(pos4, e.pos, bytes, hipMemcpyDeviceToHost, e.stream));
+    if (vel4) NB_HIP(hipMemcpyAsync(vel4, e.vel, bytes, hipMemcpyDeviceToHost, e.stream));
+    e.sync();
+  });
+}
+
+int nb_download_acc(nb_engine* h, float* acc4) {
+  return guarded([&] {
+    need(h, "engine");
+    need(acc4, "acc4");
+    nb::Engine& e = *h->e;
+    NB_HIP(hipMemcpyAsync(acc4, e.acc, (size_t)e.n * sizeof(float4), hipMemcpyDeviceToHost,
+                          e.stream));
+    e.sync();
+  });
+}
+
+int nb_compute_forces(nb_engine* h, float* acc4) {
+  return guarded([&] {
+    need(h, "engine");
+    nb::Engine& e = *h->e;
+    e.force_pass(false);
+    if (acc4)
+      NB_HIP(hipMemcpyAsync(acc4, e.acc, (size_t)e.n * sizeof(float4), hipMemcpyDeviceToHost,
+                            e.stream));
+    e.sync();
+  });
+}
+
+int nb_step(nb_engine* h, int nsteps) {
+  return guarded([&] {
+    need(h, "engine");
+    h->e->step(nsteps);
+  });
+}
+
+int nb_synchronize(nb_engine* h) {
+  return guarded([&] {
+    need(h, "engine");
+    h->e->sync();
+  });
+}
+
+int nb_total_energy(nb_engine* h, double out[3]) {
+  return guarded([&] {
+    need(h, "engine");
+    need(out, "out");
+    h->e->energy(out);
+  });
+}
+
+double nb_force_ms_total(nb_engine* h) {
+  if (!h) return 0.0;
+  if (guarded([&] { h->e->sync(); }) != 0) return -1.0;
+  return h->e->force_ms;
+}
+
+long nb_force_calls(nb_engine* h) {
+  if (!h) return 0;
+  if (guarded([&] { h->e->sync(); }) != 0) return -1;
+  return h->e->force_calls;
+}
+
+void nb_reset_force_timing(nb_engine* h) {
+  if (!h) return;
+  (void)guarded([&] { h->e->reset_timing(); });
+}
+
+int nb_set_phase_timing(nb_engine* h, int on) {
+  return guarded([&] {
+    need(h, "engine");
+    h->e->resolve();
+    h->e->phase_timing = on != 0;
+  });
+}
+
+int nb_bh_phase_ms(nb_engine* h, double out[6], long* calls) {
+  return guarded([&] {
+    need(h, "engine");
+    h->e->sync();
+    if (out)
+      for (int k = 0; k < nb::kPhases; ++k) out[k] = h->e->phase_ms[k];
+    if (calls) *calls = h->e->bh_calls;
+  });
+}
+
+int nb_bh_release(nb_engine* h) {
+  return guarded([&] {
+    need(h, "engine");
+    NB_HIP(hipStreamSynchronize(h->e->stream));
+    h->e->release_bh();
+  });
+}
+
+int nb_time_steps(nb_engine* h, int nsteps, double* ms) {
+  return guarded([&] {
+    need(h, "engine");
+    need(ms, "ms");
+    nb::Engine& e = *h->e;
+    if (nsteps > 0 && !e.acc_valid) e.force_pass(false);
+    hipEvent_t a, b;
+    NB_HIP(hipEventCreate(&a));
+    NB_HIP(hipEventCreate(&b));
+    NB_HIP(hipEventRecord(a, e.stream));
+    e.step(nsteps);
+    NB_HIP(hipEventRecord(b, e.stream));
+    NB_HIP(hipEventSynchronize(b));
+    float t = 0.f;
+    NB_HIP(hipEventElapsedTime(&t, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *ms = t;
+    e.sync();
+  });
+}
+
+int nb_get_tree(nb_engine* h, uint64_t* keys_sorted, uint32_t* perm, int32_t* children,
+                int32_t* parent, float* com4, float* size2, float* pos_sorted4) {
+  return guarded([&] {
+    need(h, "engine");
+    nb::Engine& e = *h->e;
+    if (!e.tree_valid) throw nb::Error(nb::kErrState, "no Barnes-Hut tree has been built");
+    e.sync();
+    const int n = e.n;
+    if (keys_sorted) {
+      if (e.p.key_bits == 30) {
+        std::vector<uint32_t> k(n);
+        NB_HIP(hipMemcpy(k.data(), e.keys_a.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i) keys_sorted[i] = k[i];
+      } else {
+        NB_HIP(hipMemcpy(keys_sorted, e.keys_a.p, n * sizeof(uint64_t), hipMemcpyDeviceToHost));
+      }
+    }
+    if (perm) NB_HIP(hipMemcpy(perm, e.vals_a.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (parent)
+      NB_HIP(hipMemcpy(parent, e.parent.p, (2 * (size_t)n - 1) * sizeof(int), hipMemcpyDeviceToHost));
+    if (pos_sorted4)
+      NB_HIP(hipMemcpy(pos_sorted4, e.pos_sorted.p, n * sizeof(float4), hipMemcpyDeviceToHost));
+    if (children || com4 || size2) {
+      std::vector<nb::Node> nd(n - 1);
+      NB_HIP(hipMemcpy(nd.data(), e.nodes.p, (n - 1) * sizeof(nb::Node), hipMemcpyDeviceToHost));
+      for (int i = 0; i < n - 1; ++i) {
+        if (children) {
+          children[2 * i] = nd[i].left;
+          children[2 * i + 1] = nd[i].right;
+        }
+        if (com4) {
+          com4[4 * i] = nd[i].com.x;
+          com4[4 * i + 1] = nd[i].com.y;
+          com4[4 * i + 2] = nd[i].com.z;
+          com4[4 * i + 3] = nd[i].com.w;
+        }
+        if (size2) size2[i] = nd[i].size2;
+      }
+    }
+  });
+}
+
+int nb_traverse_counts(nb_engine* h, int traverse, int group_width, int32_t* interactions,
+                       int32_t* opens) {
+  return guarded([&] {
+    need(h, "engine");
+    need(interactions, "interactions");
+    nb::Engine& e = *h->e;
+    if (!e.tree_valid) throw nb::Error(nb::kErrState, "no Barnes-Hut tree has been built");
+    if (traverse != 0 && traverse != 1) throw nb::Error(nb::kErrInvalid, "bad traverse");
+    if (traverse == 1 && group_width != 16 && group_width != 32 && group_width != 64)
+      throw nb::Error(nb::kErrInvalid, "group_width must be 16, 32 or 64");
+    const int n = e.n;
+    const int slots = traverse == 0 ? n : (n + group_width - 1) / group_width;
+    nb::DevBuf<int> ci, co;
+    ci.alloc(slots, true);
+    co.alloc(slots, true);
+    nb::WalkArgs w{};
+    w.pos_sorted = e.pos_sorted.p;
+    w.perm = e.vals_a.p;
+    w.n = n;
+    w.nodes = e.nodes.p;
+    w.theta2 = e.p.theta * e.p.theta;
+    w.eps2 = e.p.eps * e.p.eps;
+    w.G = e.p.G;
+    w.counts = ci.p;
+    w.opens = co.p;
+    w.overflow = &e.state.p->overflow;
+    w.error = &e.state.p->error;
+    nb::launch_walk(e.stream, traverse, group_width, w);
+    e.sync();
+    NB_HIP(hipMemcpy(interactions, ci.p, slots * sizeof(int), hipMemcpyDeviceToHost));
+    if (opens) NB_HIP(hipMemcpy(opens, co.p, slots * sizeof(int), hipMemcpyDeviceToHost));
+  });
+}
+
+int nb_morton(const float* pos4, int n, int key_bits, uint64_t* keys) {
+  return guarded([&] {
+    need(pos4, "pos4");
+    need(keys, "keys");
+    if (n < 1) throw nb::Error(nb::kErrInvalid, "n must be >= 1");
+    if (key_bits != 30 && key_bits != 63) throw nb::Error(nb::kErrInvalid, "key_bits must be 30 or 63");
+    nb::check_device();
+    nb::DevBuf<float4> dpos, part;
+    nb::DevBuf<uint64_t> dkeys;
+    nb::DevBuf<nb::QuantParams> qp;
+    nb::DevBuf<nb::SortState> ss;
+    nb::upload(dpos, reinterpret_cast<const float4*>(pos4), (size_t)n);
+    dkeys.alloc(n);
+    part.alloc(2 * nb::kBboxBlocks);
+    qp.alloc(1, true);
+    ss.alloc(1, true);
+    const int passes = nb::sort_passes(key_bits);
+    nb::launch_bbox(nullptr, dpos.p, n, part.p, ss.p, qp.p, passes);
+    nb::launch_morton_hist(nullptr, dpos.p, n, qp.p, key_bits, dkeys.p, ss.p);
+    NB_HIP(hipDeviceSynchronize());
+    if (key_bits == 30) {
+      std::vector<uint32_t> k(n);
+      NB_HIP(hipMemcpy(k.data(), dkeys.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+      for (int i = 0; i < n; ++i) keys[i] = k[i];
+    } else {
+      NB_HIP(hipMemcpy(keys, dkeys.p, n * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    }
+  });
+}
+
+int nb_sort_pairs(const uint64_t* keys, int n, int key_bits, uint64_t* keys_sorted,
+                  uint32_t* perm) {
+  return guarded([&] {
+    need(keys, "keys");
+    if (n < 1) throw nb::Error(nb::kErrInvalid, "n must be >= 1");
+    if (key_bits != 30 && key_bits != 63) throw nb::Error(nb::kErrInvalid, "key_bits must be 30 or 63");
+    nb::check_device();
+    const int passes = nb::sort_passes(key_bits);
+    const int tiles = (n + nb::kSortTile - 1) / nb::kSortTile;
+    nb::DevBuf<uint64_t> ka, kb;
+    nb::DevBuf<unsigned> va, vb;
+    nb::DevBuf<nb::SortState> ss;
+    nb::DevBuf<unsigned long long> lb;
+    ka.alloc(n);
+    kb.alloc(n);
+    va.alloc(n);
+    vb.alloc(n);
+    ss.alloc(1, true);
+    lb.alloc((size_t)nb::kMaxPasses * tiles * 256, true);
+    if (key_bits == 30) {
+      std::vector<uint32_t> k(n);
+      for (int i = 0; i < n; ++i) {
+        if (keys[i] >> 30) throw nb::Error(nb::kErrInvalid, "30-bit sort given a key >= 2^30");
+        k[i] = (uint32_t)keys[i];
+      }
+      NB_HIP(hipMemcpy(ka.p, k.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    } else {
+      NB_HIP(hipMemcpy(ka.p, keys, n * sizeof(uint64_t), hipMemcpyHostToDevice));
+    }
+    nb::launch_sort_init(nullptr, ss.p, passes);
+    nb::launch_hist(nullptr, ka.p, n, key_bits, ss.p);
+    nb::launch_sort(nullptr, key_bits, ka.p, kb.p, va.p, vb.p, n, ss.p, lb.p, tiles, true,
+                    nullptr, nullptr);
+    NB_HIP(hipDeviceSynchronize());
+    if (keys_sorted) {
+      if (key_bits == 30) {
+        std::vector<uint32_t> k(n);
+        NB_HIP(hipMemcpy(k.data(), ka.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i) keys_sorted[i] = k[i];
+      } else {
+        NB_HIP(hipMemcpy(keys_sorted, ka.p, n * sizeof(uint64_t), hipMemcpyDeviceToHost));
+      }
+    }
+    if (perm) NB_HIP(hipMemcpy(perm, va.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  });
+}
+
+int nb_karras_build(const uint64_t* keys_sorted, int n, int key_bits, int32_t* children,
+                    int32_t* parent) {
+  return guarded([&] {
+    need(keys_sorted, "keys_sorted");
+    if (n < 2) throw nb::Error(nb::kErrInvalid, "n must be >= 2");
+    if (key_bits != 30 && key_bits != 63) throw nb::Error(nb::kErrInvalid, "key_bits must be 30 or 63");
+    nb::check_device();
+    nb::DevBuf<uint64_t> dk;
+    nb::DevBuf<nb::Node> nodes;
+    nb::DevBuf<int> par;
+    nb::DevBuf<unsigned> visit;
+    dk.alloc(n);
+    if (key_bits == 30) {
+      std::vector<uint32_t> k(n);
+      for (int i = 0; i < n; ++i) k[i] = (uint32_t)keys_sorted[i];
+      NB_HIP(hipMemcpy(dk.p, k.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    } else {
+      NB_HIP(hipMemcpy(dk.p, keys_sorted, n * sizeof(uint64_t), hipMemcpyHostToDevice));
+    }
+    nodes.alloc(n - 1);
+    par.alloc(2 * (size_t)n - 1);
+    visit.alloc(n - 1);
+    nb::launch_karras(nullptr, key_bits, dk.p, n, nodes.p, par.p, visit.p);
+    NB_HIP(hipDeviceSynchronize());
+    if (parent)
+      NB_HIP(hipMemcpy(parent, par.p, (2 * (size_t)n - 1) * sizeof(int), hipMemcpyDeviceToHost));
+    if (children) {
+      std::vector<nb::Node> nd(n - 1);
+      NB_HIP(hipMemcpy(nd.data(), nodes.p, (n - 1) * sizeof(nb::Node), hipMemcpyDeviceToHost));
+      for (int i = 0; i < n - 1; ++i) {
+        children[2 * i] = nd[i].left;
+        children[2 * i + 1] = nd[i].right;
+      }
+    }
+  });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------
+// Reference-named C++ API (include/nbody/*.hpp).  One process-global engine
+// bound to the caller's device arrays, on the null stream; errors abort with
+// the HIP message, as the reference's CUDA_CHECK does.
+
+namespace {
+
+[[noreturn]] void die_api(const char* where, const std::exception& e) {
+  std::fprintf(stderr, "HIP error in %s: %s\n", where, e.what());
+  std::exit(EXIT_FAILURE);
+}
+
+template <typename F>
+void api(const char* where, F&& body) {
+  try {
+    body();
+  } catch (const std::exception& e) {
+    die_api(where, e);
+  }
+}
+
+nb::Engine* g_engine = nullptr;
+
+nb::Engine& global_engine() {
+  if (!g_engine) g_engine = new nb::Engine();
+  return *g_engine;
+}
+
+nb::Params to_params(const SimParams& sp) {
+  nb::Params p;
+  p.dt = sp.dt;
+  p.eps = sp.epsilon;
+  p.G = sp.G;
+  p.theta = sp.theta;
+  p.force = sp.force == ForceMethod::kBarnesHut ? 1 : 0;
+  p.traverse = sp.traverse == BhTraversal::kWarp ? 1 : 0;
+  p.key_bits = sp.key_bits;
+  p.group_width = sp.group_width;
+  p.precision = sp.compensated ? 1 : 0;
+  return p;
+}
+
+void run_forces(const ParticleSystem& sys, float4* d_acc, const SimParams& sp, int force,
+                bool timed) {
+  nb::Engine& e = global_engine();
+  nb::Params p = to_params(sp);
+  p.force = force;
+  e.set_params(p);
+  e.bind(sys.d_pos, sys.d_vel, d_acc, sys.n);
+  e.timing_on = timed;
+  e.force_pass(false);
+  e.timing_on = true;
+  e.sync();
+}
+
+}  // namespace
+
+void ParticleSystem::allocate(int count) {
+  api("ParticleSystem::allocate", [&] {
+    if (count < 1) throw nb::Error(nb::kErrInvalid, "particle count must be >= 1");
+    n = count;
+    NB_HIP(hipMalloc(reinterpret_cast<void**>(&d_pos), n * sizeof(float4)));
+    NB_HIP(hipMalloc(reinterpret_cast<void**>(&d_vel), n * sizeof(float4)));
+  });
+}
+
+void ParticleSystem::release() {
+  api("ParticleSystem::release", [&] {
+    if (d_pos) NB_HIP(hipFree(d_pos));
+    if (d_vel) NB_HIP(hipFree(d_vel));
+    d_pos = d_vel = nullptr;
+    n = 0;
+  });
+}
+
+void ParticleSystem::upload(const float4* h_pos, const float4* h_vel) {
+  api("ParticleSystem::upload", [&] {
+    NB_HIP(hipMemcpy(d_pos, h_pos, n * sizeof(float4), hipMemcpyHostToDevice));
+    NB_HIP(hipMemcpy(d_vel, h_vel, n * sizeof(float4), hipMemcpyHostToDevice));
+  });
+}
+
+void ParticleSystem::download_positions(float4* h_pos) const {
+  api("ParticleSystem::download_positions", [&] { NB_HIP(hipMemcpy(h_pos, d_pos, n * sizeof(float4), hipMemcpyDeviceToHost)); });
+}
+
+void compute_forces(const ParticleSystem& sys, float4* d_acc, const SimParams& p) {
+  api("compute_forces", [&] { run_forces(sys, d_acc, p, p.force == ForceMethod::kBarnesHut ? 1 : 0, true); });
+}
+
+void forces_naive(const ParticleSystem& sys, float4* d_acc, const SimParams& p) {
+  api("forces_naive", [&] { run_forces(sys, d_acc, p, 0, false); });
+}
+
+void forces_barnes_hut(const ParticleSystem& sys, float4* d_acc, const SimParams& p) {
+  api("forces_barnes_hut", [&] { run_forces(sys, d_acc, p, 1, false); });
+}
+
+double force_kernel_ms_total() {
+  double v = 0.0;
+  api("force_kernel_ms_total", [&] {
+    global_engine().sync();
+    v = global_engine().force_ms;
+  });
+  return v;
+}
+
+long force_kernel_calls() {
+  long v = 0;
+  api("force_kernel_calls", [&] {
+    global_engine().sync();
+    v = global_engine().force_calls;
+  });
+  return v;
+}
+
+void reset_force_timing() {
+  api("reset_force_timing", [&] { global_engine().reset_timing(); });
+}
+
+void bh_report_phase_timing() {
+  api("bh_report_phase_timing", [&] {
+    nb::Engine& e = global_engine();
+    e.sync();
+    if (e.bh_calls == 0) return;
+    double total = 0.0;
+    for (int i = 0; i < nb::kPhases; ++i) total += e.phase_ms[i];
+    std::printf("barnes-hut pipeline, per-call average over %ld calls:\n", e.bh_calls);
+    for (int i = 0; i < nb::kPhases; ++i)
+      std::printf("  %-9s %8.3f ms  (%4.1f%%)\n", nb::kPhaseNames[i], e.phase_ms[i] / e.bh_calls,
+                  100.0 * e.phase_ms[i] / total);
+  });
+}
+
+void bh_release() {
+  api("bh_release", [&] {
+    if (g_engine) {
+      NB_HIP(hipDeviceSynchronize());
+      g_engine->release_bh();
+    }
+  });
+}
+
+void leapfrog_step(ParticleSystem& sys, float4* d_acc, const SimParams& sp) {
+  api("leapfrog_step", [&] {
+    nb::Engine& e = global_engine();
+    e.set_params(to_params(sp));
+    e.bind(sys.d_pos, sys.d_vel, d_acc, sys.n);
+    e.acc_valid = true;  // the caller's contract: d_acc holds a(x)
+    e.force_pass(true);
+  });
+}
+
+Energy total_energy(const ParticleSystem& sys, const SimParams& sp) {
+  Energy out{0.0, 0.0, 0.0};
+  api("total_energy", [&] {
+    nb::Engine& e = global_engine();
+    e.set_params(to_params(sp));
+    e.bind(sys.d_pos, sys.d_vel, e.acc, sys.n);
+    double r[3];
+    e.energy(r);
+    out = Energy{r[0], r[1], r[2]};
+  });
+  return out;
+}

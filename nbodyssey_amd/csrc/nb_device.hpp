@@ -1,0 +1,89 @@
+// Device helpers shared by the kernels: the canonical FP32 force law and
+// wave64 reductions.  Everything is compiled with -ffp-contract=off, so an FMA
+// appears only where fmaf is written; the CPU oracle uses the same order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace nb {
+
+__device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// |d|^2 without softening, in the canonical order shared with the oracle.
+__device__ __forceinline__ float dist2(float dx, float dy, float dz) {
+  return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+}
+
+// Softened point-mass acceleration of one source (forces.cu:17-31), given the
+// separation d = src - target and its d2 = dist2(d).
+__device__ __forceinline__ void accumulate(float dx, float dy, float dz, float d2, float m,
+                                           float eps2, float& ax, float& ay, float& az) {
+  float inv = rsq(d2 + eps2);
+  float f = m * inv * inv * inv;
+  ax = fmaf(dx, f, ax);
+  ay = fmaf(dy, f, ay);
+  az = fmaf(dz, f, az);
+}
+
+// Kahan-compensated add of `x` into (sum, c).
+__device__ __forceinline__ void kahan_add(float& sum, float& c, float x) {
+  float y = x - c;
+  float t = sum + y;
+  c = (t - sum) - y;
+  sum = t;
+}
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+template <typename T>
+__device__ __forceinline__ T wave_min(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Inclusive prefix sum over the 64 lanes of a wave.
+__device__ __forceinline__ unsigned wave_incl_scan(unsigned v) {
+  int lane = lane_id();
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    unsigned u = __shfl_up(v, o, 64);
+    if (lane >= o) v += u;
+  }
+  return v;
+}
+
+// Agent-scope atomics (visible across all XCDs of the device).
+__device__ __forceinline__ unsigned atomic_add_agent(unsigned* p, unsigned v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned atomic_add_relaxed(unsigned* p, unsigned v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_agent(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long load_agent(unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_agent_u32(unsigned* p, unsigned v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned load_agent_u32(unsigned* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+}  // namespace nb

@@ -1,0 +1,143 @@
+// Internal declarations shared by the HIP translation units of libnbody.so.
+//
+// Device data layout (HBM):
+//   pos, vel, acc        float4 x n, original particle order (simulation.hpp:17-28)
+//   keys_a/keys_b        u32 (30-bit keys) or u64 (63-bit keys) x n, sort double buffer
+//   vals_a/vals_b        u32 x n, original index riding with each key
+//   pos_sorted           float4 x n, positions gathered into Morton order
+//   nodes                Node (32 B) x (n-1), one record per internal node
+//   parent               i32 x (2n-1)
+//   visit                u32 x (n-1), bottom-up arrival counters
+//   box_lo/box_hi        float4 x (n-1), internal-node AABBs (used by the COM pass only)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace nb {
+
+// Error raised inside the library; the C ABI turns it into a status code and
+// a message (the reference aborts instead, cuda_check.cuh:7-15).
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+enum Status {
+  kOk = 0,
+  kErrInvalid = 1,
+  kErrDevice = 2,
+  kErrNoMem = 3,
+  kErrState = 4,
+  kErrOverflow = 5,
+};
+
+#define NB_HIP(call)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      throw ::nb::Error(e_ == hipErrorOutOfMemory ? ::nb::kErrNoMem : ::nb::kErrDevice, \
+                        std::string(#call) + ": " + hipGetErrorString(e_));           \
+  } while (0)
+
+#define NB_LAUNCH_CHECK() NB_HIP(hipGetLastError())
+
+constexpr int kBlock = 256;          // kForceBlock (forces.cuh:25)
+constexpr int kStackDepth = 128;     // kBhStack (forces_bh.cu:43)
+constexpr int kBboxBlocks = 512;     // grid of the bbox / kick-drift-bbox reductions
+constexpr int kSortItems = 16;       // keys per thread per onesweep tile
+constexpr int kSortTile = kBlock * kSortItems;
+constexpr int kMaxPasses = 8;
+
+// One internal tree node, 32 bytes so a wave-uniform walk fetches it with a
+// single scalar load.  com = (x, y, z, mass); size2 = (longest AABB edge)^2.
+struct alignas(32) Node {
+  float4 com;
+  int left;
+  int right;
+  float size2;
+  int pad;
+};
+
+// Morton quantisation constants, formed in FP64 by the bbox reduction.
+struct QuantParams {
+  double lo[3];
+  double inv;
+  float box_lo[4];
+  float box_hi[4];
+};
+
+// Onesweep bookkeeping.  `epoch` tags look-back words so they never need
+// clearing; `ticket` hands out tile ids in dispatch order.
+struct SortState {
+  unsigned hist[kMaxPasses][256];
+  unsigned ticket[kMaxPasses];
+  unsigned epoch;
+  unsigned bbox_ticket;
+  int overflow;             // traversal stack overflow flag
+  int error;                // consistency guard tripped (bad sort/tree data)
+  int pad[3];
+};
+
+// ---------------------------------------------------------------- launchers
+// Every launcher enqueues on `s` and checks the launch.
+
+// k_integrate.hip
+void launch_kick_drift(hipStream_t s, float4* pos, float4* vel, const float4* acc, int n,
+                       float half_dt, float dt, bool with_bbox, float4* partial,
+                       SortState* ss, QuantParams* qp, int passes);
+void launch_bbox(hipStream_t s, const float4* pos, int n, float4* partial, SortState* ss,
+                 QuantParams* qp, int passes);
+void launch_kick(hipStream_t s, float4* vel, const float4* acc, int n, float half_dt);
+void launch_fill_u32(hipStream_t s, unsigned* p, unsigned v, size_t count);
+
+// k_allpairs.hip
+int allpairs_split(int n);
+void launch_allpairs(hipStream_t s, const float4* pos, int n, float eps2, float G,
+                     float4* acc, float4* partial, int split, bool compensated,
+                     float4* vel_kick, float half_dt);
+void launch_energy(hipStream_t s, const float4* pos, const float4* vel, int n, float eps2,
+                   float G, double* partial, int max_partial, double* out);
+int energy_partials(int n);
+
+// k_lbvh.hip
+void launch_morton_hist(hipStream_t s, const float4* pos, int n, const QuantParams* qp,
+                        int key_bits, void* keys, SortState* ss);
+void launch_hist(hipStream_t s, const void* keys, int n, int key_bits, SortState* ss);
+void launch_sort_init(hipStream_t s, SortState* ss, int passes);
+// Sorts (keys_a, vals) in place through the b buffers; result lands in *_a for
+// an even number of passes.  If gather_src is set, the last pass also writes
+// gather_dst[dest] = gather_src[val].  first_pass_identity: vals_a is taken as
+// the index (not read).
+void launch_sort(hipStream_t s, int key_bits, void* keys_a, void* keys_b, unsigned* vals_a,
+                 unsigned* vals_b, int n, SortState* ss, unsigned long long* lookback,
+                 int max_tiles, bool first_pass_identity, const float4* gather_src,
+                 float4* gather_dst);
+// (the sort sets SortState::error if a tile id or destination is out of range)
+int sort_passes(int key_bits);
+void launch_karras(hipStream_t s, int key_bits, const void* keys, int n, Node* nodes, int* parent,
+                   unsigned* visit);
+void launch_com(hipStream_t s, const float4* pos_sorted, int n, Node* nodes, const int* parent,
+                unsigned* visit, float4* box_lo, float4* box_hi, int* error);
+
+// k_traverse.hip
+struct WalkArgs {
+  const float4* pos_sorted;
+  const unsigned* perm;
+  int n;
+  const Node* nodes;
+  float theta2, eps2, G;
+  float4* acc;        // output, original order (may be null in counting mode)
+  float4* vel;        // if set: vel[perm[t]] += half_dt * acc (closing kick)
+  float half_dt;
+  int* counts;        // per target (thread walk) or per group: interactions
+  int* opens;         // same, opening tests
+  int* overflow;      // set to 1 if a stack would overflow
+  int* error;         // set to 1 if a node id is out of range or the walk is too long
+};
+void launch_walk(hipStream_t s, int traverse, int group_width, const WalkArgs& a);
+
+}  // namespace nb

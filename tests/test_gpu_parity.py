@@ -1,0 +1,256 @@
+"""GPU parity: libnbody.so (HIP, gfx950) against the CPU oracle.
+
+Integer/index/topology results and FP32 node values are bit-exact (the oracle
+and the kernels share one canonical operation order); accelerations differ only
+by the hardware rsqrt (v_rsq_f32, ~1 ulp) and summation order, so they are
+compared with the tolerances written in each test.
+"""
+import numpy as np
+import pytest
+
+from nbodyssey_amd import engine, ic
+from oracle import orc
+
+pytestmark = pytest.mark.gpu
+
+
+def blobs_with_dupes(seed=1):
+    """The input of lbvh_check.py:220-229: two Gaussian blobs of 1,500 points
+    and 24 coincident points (FP64 there, FP32 here), mass 1/n."""
+    rng = np.random.default_rng(seed)
+    a = rng.normal([-2.0, 0.0, 0.0], 0.7, size=(1500, 3))
+    b = rng.normal([2.0, 0.5, 0.1], 0.7, size=(1500, 3))
+    d = np.tile([[0.123, -0.456, 0.789]], (24, 1))
+    p = np.concatenate([a, b, d])
+    return np.column_stack([p, np.full(len(p), 1.0 / len(p))]).astype(np.float32)
+
+
+def rel_err(a, b):
+    a = np.asarray(a, np.float64)[:, :3]
+    b = np.asarray(b, np.float64)[:, :3]
+    return np.linalg.norm(a - b, axis=1) / (np.linalg.norm(b, axis=1) + 1e-30)
+
+
+CASES = {
+    "blobs3024": lambda: blobs_with_dupes(),
+    "disk65536": lambda: ic.two_disk_encounter(65536, seed=1, eps=0.02)[0],
+    "disk5000": lambda: ic.two_disk_encounter(5000, seed=7)[0],
+    "n1": lambda: np.array([[0.5, -1.0, 2.0, 1.0]], np.float32),
+    "n2": lambda: np.array([[0, 0, 0, 1], [1, 0, 0, 1]], np.float32),
+    "n3": lambda: np.array([[0, 0, 0, 1], [1, 2, 0, 0.5], [-1, 0.5, 3, 2]], np.float32),
+    "same": lambda: np.tile(np.array([[0.3, 0.3, 0.3, 0.01]], np.float32), (777, 1)),
+}
+
+
+@pytest.mark.parametrize("bits", [63, 30])
+@pytest.mark.parametrize("case", list(CASES))
+def test_morton_bit_exact(case, bits):
+    pos = CASES[case]()
+    print(f"morton case={case} bits={bits} n={len(pos)}", flush=True)
+    assert np.array_equal(engine.morton_keys(pos, bits), orc.morton(pos, bits))
+
+
+@pytest.mark.parametrize("bits", [63, 30])
+@pytest.mark.parametrize("n", [1, 2, 100, 4095, 4096, 4097, 65537, 300001])
+def test_sort_bit_exact(n, bits):
+    rng = np.random.default_rng(n + bits)
+    hi = (1 << 30) if bits == 30 else (1 << 63)
+    # few distinct values -> long runs of equal keys (stability matters)
+    keys = rng.integers(0, hi, size=max(n // 7, 1), dtype=np.uint64)[rng.integers(0, max(n // 7, 1), n)]
+    print(f"sort n={n} bits={bits}", flush=True)
+    ks, perm = engine.sort_pairs(keys, bits)
+    rks, rperm = orc.sort_pairs(keys)
+    assert np.array_equal(ks, rks)
+    assert np.array_equal(perm, rperm)
+
+
+@pytest.mark.parametrize("bits", [63, 30])
+@pytest.mark.parametrize("case", ["blobs3024", "disk65536", "disk5000", "n2", "n3", "same"])
+def test_karras_bit_exact(case, bits):
+    pos = CASES[case]()
+    ks, _ = orc.sort_pairs(orc.morton(pos, bits))
+    print(f"karras case={case} bits={bits} n={len(pos)}", flush=True)
+    ch, par = engine.build_tree(ks, bits)
+    rch, rpar = orc.karras(ks, bits)
+    assert np.array_equal(ch, rch)
+    assert np.array_equal(par, rpar)
+
+
+@pytest.mark.parametrize("bits", [63, 30])
+@pytest.mark.parametrize("case", ["blobs3024", "disk65536", "n2", "n3", "same"])
+def test_tree_pipeline_bit_exact(case, bits):
+    """bbox -> Morton -> sort -> gather -> Karras -> COM inside one force call."""
+    pos = CASES[case]()
+    print(f"tree case={case} bits={bits} n={len(pos)}", flush=True)
+    with engine.Simulation(pos, eps=0.05, theta=0.5, force="bh", traverse="thread",
+                           key_bits=bits) as sim:
+        sim.compute_forces()
+        t = sim.tree()
+    ref = orc.Tree(pos, key_bits=bits)
+    assert np.array_equal(t["keys"], ref.keys_sorted)
+    assert np.array_equal(t["perm"], ref.perm)
+    assert np.array_equal(t["pos_sorted"], ref.pos_sorted)
+    assert np.array_equal(t["children"], ref.children)
+    assert np.array_equal(t["parent"], ref.parent)
+    assert np.array_equal(t["com4"], ref.com4)
+    assert np.array_equal(t["size2"], ref.size2)
+
+
+@pytest.mark.parametrize("theta", [0.0, 0.5, 0.8])
+@pytest.mark.parametrize("walk", [("thread", 1), ("warp", 16), ("warp", 32), ("warp", 64)])
+@pytest.mark.parametrize("case", ["blobs3024", "disk5000"])
+def test_walk_counts_and_acc(case, walk, theta):
+    """Interaction/opening counts are exact; accelerations agree to rsqrt
+    rounding (median 1e-6, max 1e-4 relative)."""
+    pos = CASES[case]()
+    traverse, width = walk
+    print(f"walk case={case} walk={walk} theta={theta}", flush=True)
+    with engine.Simulation(pos, eps=0.05, theta=theta, force="bh", traverse=traverse,
+                           group_width=max(width, 16), key_bits=63) as sim:
+        acc = sim.compute_forces()
+        inter, opens = sim.traverse_counts(traverse, max(width, 16))
+    ref = orc.Tree(pos, key_bits=63)
+    if traverse == "thread":
+        racc, _, ri, ro, _ = ref.traverse_thread(theta, 0.05)
+    else:
+        racc, _, ri, ro, _ = ref.traverse_group(theta, 0.05, width)
+    assert np.array_equal(inter, ri)
+    assert np.array_equal(opens, ro)
+    e = rel_err(acc, racc)
+    assert np.median(e) < 1e-6 and e.max() < 1e-4, (np.median(e), e.max())
+    if theta == 0.0:
+        assert np.all(ri == len(pos)), "theta=0 must touch every leaf"
+
+
+@pytest.mark.parametrize("walk", [("thread", 32), ("warp", 32), ("warp", 64)])
+def test_theta0_matches_allpairs_65536(walk):
+    """theta = 0 BH reproduces all-pairs to FP32 round-off (the reference's
+    gate: mean < 1e-4, gpu_bench.sh:43; here mean < 1e-5, max < 1e-3)."""
+    pos, _ = ic.two_disk_encounter(65536, seed=1, eps=0.02)
+    with engine.Simulation(pos, eps=0.02, theta=0.0, force="naive") as sim:
+        a = sim.compute_forces()
+        sim.update(force="bh", traverse=walk[0], group_width=walk[1], key_bits=30)
+        b = sim.compute_forces()
+    e = rel_err(b, a)
+    print(f"theta0 {walk}: mean {e.mean():.2e} max {e.max():.2e}", flush=True)
+    assert e.mean() < 1e-5 and e.max() < 1e-3
+
+
+def test_allpairs_vs_oracle_12k():
+    """Naive kernel vs the oracle's FP32 j-order restatement of force_kernel
+    (median <= 1e-5 relative) and vs FP64."""
+    pos, _ = ic.two_disk_encounter(12000, seed=0, eps=0.05)
+    with engine.Simulation(pos, eps=0.05, force="naive") as sim:
+        a = sim.compute_forces()
+        sim.update(compensated=True)
+        ac = sim.compute_forces()
+    r32 = orc.allpairs_f32(pos, 0.05)
+    r64 = orc.allpairs_f64(pos, 0.05)
+    e32, e64, ec = rel_err(a, r32), rel_err(a, r64), rel_err(ac, r64)
+    print(f"allpairs: vs f32 med {np.median(e32):.2e}, vs f64 med {np.median(e64):.2e}, "
+          f"compensated vs f64 med {np.median(ec):.2e}", flush=True)
+    assert np.median(e32) < 1e-5 and e32.max() < 1e-3
+    assert np.median(e64) < 1e-5 and e64.max() < 1e-3
+    assert np.median(ec) <= np.median(e64) * 1.5
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 257, 12000])
+def test_energy_vs_oracle(n):
+    pos, vel = ic.two_disk_encounter(max(n, 2), seed=5, eps=0.05)
+    pos, vel = pos[:n], vel[:n]
+    with engine.Simulation(pos, vel, eps=0.05) as sim:
+        e = sim.total_energy()
+    r = orc.energy_f64(pos, vel, 0.05)
+    print(f"energy n={n}: gpu {e} oracle {r}", flush=True)
+    np.testing.assert_allclose(e, r, rtol=2e-6, atol=1e-12)
+
+
+@pytest.mark.parametrize("force", ["naive", "bh"])
+def test_one_step_vs_oracle(force):
+    """One fused KDK step equals the oracle's separate kick/drift/force/kick."""
+    pos, vel = ic.two_disk_encounter(4096, seed=11, eps=0.05)
+    with engine.Simulation(pos, vel, dt=0.01, eps=0.05, force=force, theta=0.0,
+                           traverse="warp", group_width=64) as sim:
+        a0 = sim.compute_forces()
+        sim.step(1)
+        p1, v1 = sim.download()
+        a1 = sim.acc()
+    p, v, a = pos.copy(), vel.copy(), a0.copy()
+    orc.kdk_step_naive(p, v, a, 0.01, 0.05)
+    # drift/kick are exact FMAs of the same inputs; differences come only from a(x)
+    np.testing.assert_allclose(p1, p, rtol=0, atol=1e-6)
+    np.testing.assert_allclose(v1, v, rtol=0, atol=2e-6)
+    assert np.median(rel_err(a1, a)) < 1e-5
+
+
+def test_energy_drift_two_disk_12k():
+    """Config-1 encounter: 12k particles, naive, dt 0.01, 200 steps; total
+    energy error stays within 0.02% (2e-4)."""
+    pos, vel = ic.two_disk_encounter(12000, seed=0, eps=0.05)
+    with engine.Simulation(pos, vel, dt=0.01, eps=0.05, force="naive") as sim:
+        e0 = sim.total_energy()
+        worst = 0.0
+        for _ in range(10):
+            sim.step(20)
+            e = sim.total_energy()
+            worst = max(worst, abs((e[0] - e0[0]) / e0[0]))
+    print(f"12k naive 200 steps: max |dE/E| {worst:.2e}", flush=True)
+    assert worst < 2e-4
+
+
+def test_reference_ic_energy_trace():
+    """The reference's committed GPU trace (benchmarks/energy_gpu_naive.csv,
+    12k generate_ic.py particles, dt 0.01, eps 0.05): E0 = -0.1178396 and the
+    row of step 100 (E -1.17839655e-01, KE 6.01599732e-01).  Same IC bytes,
+    same physics: E0 to 1e-6, KE/PE at step 100 to 2e-4 (FP32 trajectories)."""
+    pos, vel = ic.generate_ic_disks(particles=12000, seed=0)
+    with engine.Simulation(pos, vel, dt=0.01, eps=0.05, force="naive") as sim:
+        e0 = sim.total_energy()
+        sim.step(101)  # rows are written after step k (main.cu:186-192)
+        e = sim.total_energy()
+    assert abs(e0[0] / -0.1178395557 - 1) < 1e-6
+    assert abs(e[1] / 6.01599732e-01 - 1) < 2e-4
+    assert abs(e[2] / -7.19439386e-01 - 1) < 2e-4
+
+
+@pytest.mark.parametrize("walk", [("warp", 64), ("warp", 32), ("thread", 32)])
+def test_theta05_median_error_vs_fp64(walk):
+    """theta = 0.5 median force error vs FP64 all-pairs <= 1% (65,536
+    particles, 512 sampled targets)."""
+    pos, _ = ic.two_disk_encounter(65536, seed=1, eps=0.02)
+    with engine.Simulation(pos, eps=0.02, theta=0.5, force="bh", traverse=walk[0],
+                           group_width=walk[1], key_bits=30) as sim:
+        a = sim.compute_forces()
+    idx = np.random.default_rng(0).choice(len(pos), 512, replace=False).astype(np.int32)
+    r = orc.allpairs_f64(pos, 0.02, targets=idx)
+    e = rel_err(a[idx], r)
+    print(f"theta0.5 {walk}: median {np.median(e):.3e} p99 {np.quantile(e, 0.99):.3e}", flush=True)
+    assert np.median(e) < 0.01
+
+
+def test_n_small_bh_zero_force():
+    """n < 2 under Barnes-Hut gives zero acceleration (forces_bh.cu:466-469)."""
+    pos = np.array([[1.0, 2.0, 3.0, 1.0]], np.float32)
+    with engine.Simulation(pos, force="bh") as sim:
+        a = sim.compute_forces()
+        sim.step(3)
+    assert np.all(a == 0)
+
+
+def test_large_sort_properties_1m():
+    """Full benchmark size: the pipeline's sorted keys are non-decreasing, perm
+    is a permutation, ties keep ascending index, and the tree is consistent."""
+    pos, _ = ic.two_disk_encounter(1 << 20, seed=1, eps=0.02)
+    with engine.Simulation(pos, eps=0.02, theta=0.5, force="bh", traverse="warp",
+                           group_width=64, key_bits=30) as sim:
+        sim.compute_forces()
+        t = sim.tree()
+    k, perm = t["keys"], t["perm"].astype(np.int64)
+    assert np.all(k[1:] >= k[:-1])
+    assert np.array_equal(np.sort(perm), np.arange(len(perm)))
+    ties = k[1:] == k[:-1]
+    assert np.all(perm[1:][ties] > perm[:-1][ties])
+    assert np.array_equal(k, orc.morton(pos, 30)[perm])
+    n = len(pos)
+    assert t["parent"][0] == -1 and np.all(t["parent"][1:] >= 0)
+    assert abs(t["com4"][0, 3] - pos[:, 3].astype(np.float64).sum()) < 1e-5
