@@ -142,10 +142,11 @@ class Engine {
       launch_sort(stream, p.key_bits, keys_a.p, keys_b.p, vals_a.p, vals_b.p, n, state.p,
                   lookback.p, max_tiles, true, pos, pos_sorted.p);
       mark(ce, kPhSort + 1);
-      launch_karras(stream, p.key_bits, keys_a.p, n, nodes.p, parent.p, visit.p);
+      launch_karras(stream, p.key_bits, keys_a.p, n, nodes.p, parent.p, visit.p,
+                    &state.p->frontier_count);
       mark(ce, kPhBuild + 1);
       launch_com(stream, pos_sorted.p, n, nodes.p, parent.p, visit.p, box_lo.p, box_hi.p,
-                 &state.p->error);
+                 frontier.p, &state.p->frontier_count, &state.p->error);
       mark(ce, kPhCom + 1);
       WalkArgs w{};
       w.pos_sorted = pos_sorted.p;
@@ -160,6 +161,7 @@ class Engine {
       w.half_dt = half;
       w.overflow = &state.p->overflow;
       w.error = &state.p->error;
+      w.compensated = p.precision == 1;
       launch_walk(stream, p.traverse, p.group_width, w);
       mark(ce, kPhWalk + 1);
       tree_valid = true;
@@ -233,6 +235,7 @@ class Engine {
     visit.alloc(c);
     box_lo.alloc(c);
     box_hi.alloc(c);
+    frontier.alloc(c + 1);
     bh_partial.alloc(2 * kBboxBlocks);
     quant.alloc(1, true);
     state.alloc(1, true);
@@ -242,7 +245,7 @@ class Engine {
   }
   void release_bh() {
     keys_a.reset(); keys_b.reset(); vals_a.reset(); vals_b.reset(); pos_sorted.reset();
-    nodes.reset(); parent.reset(); visit.reset(); box_lo.reset(); box_hi.reset();
+    nodes.reset(); parent.reset(); visit.reset(); box_lo.reset(); box_hi.reset(); frontier.reset();
     bh_partial.reset(); quant.reset(); state.reset(); lookback.reset();
     bh_cap = 0;
     tree_valid = false;
@@ -329,7 +332,7 @@ class Engine {
   DevBuf<unsigned> vals_a, vals_b, visit;
   DevBuf<float4> pos_sorted, box_lo, box_hi, bh_partial;
   DevBuf<Node> nodes;
-  DevBuf<int> parent;
+  DevBuf<int> parent, frontier;
   DevBuf<QuantParams> quant;
   DevBuf<SortState> state;
   DevBuf<unsigned long long> lookback;
@@ -773,7 +776,7 @@ int nb_karras_build(const uint64_t* keys_sorted, int n, int key_bits, int32_t* c
     nodes.alloc(n - 1);
     par.alloc(2 * (size_t)n - 1);
     visit.alloc(n - 1);
-    nb::launch_karras(nullptr, key_bits, dk.p, n, nodes.p, par.p, visit.p);
+    nb::launch_karras(nullptr, key_bits, dk.p, n, nodes.p, par.p, visit.p, nullptr);
     NB_HIP(hipDeviceSynchronize());
     if (parent)
       NB_HIP(hipMemcpy(parent, par.p, (2 * (size_t)n - 1) * sizeof(int), hipMemcpyDeviceToHost));

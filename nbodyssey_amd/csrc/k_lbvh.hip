@@ -17,10 +17,13 @@
 // Stable: ties keep input order, so equal keys end in ascending original index.
 //
 // Karras build (forces_bh.cu:234-277): one thread per internal node.
-// COM (forces_bh.cu:281-331): one thread per leaf climbs towards the root; an
-// agent-scope acq_rel counter per node lets the second arrival combine both
-// children (the release/acquire pair makes the first child's node data visible
-// across XCDs).  The combine is FP64 from FP32 children, rounded once.
+// COM (forces_bh.cu:281-331): leaves climb towards the root and the second
+// arrival at a node combines both children.  Pass 1 does this per workgroup
+// over 4096 consecutive leaves with LDS arrival counters; pass 2 (one
+// workgroup) finishes the few nodes whose leaf range spans workgroups.  No
+// step needs device-wide release/acquire (which would write back / invalidate
+// an XCD's L2 on every climb step).  The combine is FP64 from FP32 children,
+// rounded once.
 #include "nb_device.hpp"
 #include "nb_internal.hpp"
 
@@ -228,8 +231,10 @@ template <typename K>
 __global__ __launch_bounds__(kBlock) void karras_kernel(const K* __restrict__ keys, int n,
                                                         Node* __restrict__ nodes,
                                                         int* __restrict__ parent,
-                                                        unsigned* __restrict__ visit) {
+                                                        unsigned* __restrict__ visit,
+                                                        unsigned* __restrict__ frontier_count) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0 && frontier_count) *frontier_count = 0;
   if (i >= n - 1) return;
   const K ki = keys[i];
   int d = (delta(keys, n, i, ki, i + 1) - delta(keys, n, i, ki, i - 1)) >= 0 ? 1 : -1;
@@ -251,69 +256,135 @@ __global__ __launch_bounds__(kBlock) void karras_kernel(const K* __restrict__ ke
   int left = (split == first) ? (n - 1 + split) : split;
   int right = (split + 1 == last) ? (n - 1 + split + 1) : split + 1;
   *reinterpret_cast<int2*>(&nodes[i].left) = make_int2(left, right);
+  nodes[i].other = j;
   parent[left] = i;
   parent[right] = i;
   visit[i] = 0;
   if (i == 0) parent[0] = -1;
 }
 
-__global__ __launch_bounds__(kBlock) void com_kernel(const float4* __restrict__ pos_sorted, int n,
-                                                     Node* nodes, const int* __restrict__ parent,
-                                                     unsigned* visit, float4* box_lo,
-                                                     float4* box_hi, int* error) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// Combine the two children of internal node `node` (forces_bh.cu:293-324):
+// FP64 mass-weighted centre of mass rounded once to FP32, AABB union, squared
+// longest edge.  Returns false if a child id is out of range.
+__device__ __forceinline__ bool combine_node(const float4* __restrict__ pos_sorted, int nm1,
+                                             unsigned nodes_total, Node* nodes, float4* box_lo,
+                                             float4* box_hi, int node) {
+  int2 ch = *reinterpret_cast<const int2*>(&nodes[node].left);
+  if ((unsigned)ch.x >= nodes_total || (unsigned)ch.y >= nodes_total) return false;
+  float4 cl, ll, hl, cr, lr, hr;
+  if (ch.x >= nm1) {
+    cl = pos_sorted[ch.x - nm1];
+    ll = hl = cl;
+  } else {
+    cl = nodes[ch.x].com;
+    ll = box_lo[ch.x];
+    hl = box_hi[ch.x];
+  }
+  if (ch.y >= nm1) {
+    cr = pos_sorted[ch.y - nm1];
+    lr = hr = cr;
+  } else {
+    cr = nodes[ch.y].com;
+    lr = box_lo[ch.y];
+    hr = box_hi[ch.y];
+  }
+  double ml = (double)cl.w, mr = (double)cr.w, m = ml + mr;
+  float4 c;
+  c.x = (float)(((double)cl.x * ml + (double)cr.x * mr) / m);
+  c.y = (float)(((double)cl.y * ml + (double)cr.y * mr) / m);
+  c.z = (float)(((double)cl.z * ml + (double)cr.z * mr) / m);
+  c.w = (float)m;
+  float4 lo = make_float4(fminf(ll.x, lr.x), fminf(ll.y, lr.y), fminf(ll.z, lr.z), 0.f);
+  float4 hi = make_float4(fmaxf(hl.x, hr.x), fmaxf(hl.y, hr.y), fmaxf(hl.z, hr.z), 0.f);
+  float s = fmaxf(hi.x - lo.x, fmaxf(hi.y - lo.y, hi.z - lo.z));
+  nodes[node].com = c;
+  nodes[node].size2 = s * s;
+  box_lo[node] = lo;
+  box_hi[node] = hi;
+  return true;
+}
+
+// Pass 1: workgroup b owns the sorted leaves [b*L, b*L + L).  Internal node i
+// covers the leaves between i and nodes[i].other, so a node whose range lies
+// inside the workgroup's leaves has its id in [lo, hi] and is reached only by
+// this workgroup: its arrival counter lives in LDS.  A climb that reaches a
+// node spanning workgroups stops and records the finished child in
+// `frontier` for pass 2.
+constexpr int kComThreads = 1024;
+constexpr int kComLeaves = 4096;
+
+__global__ __launch_bounds__(kComThreads) void com_local_kernel(
+    const float4* __restrict__ pos_sorted, int n, Node* nodes, const int* __restrict__ parent,
+    float4* box_lo, float4* box_hi, int* frontier, unsigned* frontier_count, int* error) {
+  __shared__ unsigned s_visit[kComLeaves];
+  for (int k = threadIdx.x; k < kComLeaves; k += blockDim.x) s_visit[k] = 0;
+  __syncthreads();
   const int nm1 = n - 1;
   const unsigned nodes_total = 2u * (unsigned)n - 1u;
-  int node = parent[nm1 + i];
-  while (node >= 0) {
-    if (node >= nm1) {  // a parent must be internal
+  const int lo = blockIdx.x * kComLeaves;
+  const int hi = min(n - 1, lo + kComLeaves - 1);
+  for (int leaf = lo + (int)threadIdx.x; leaf <= hi; leaf += blockDim.x) {
+    int child = nm1 + leaf;
+    int node = parent[child];
+    while (node >= 0) {
+      if (node >= nm1) {
+        *error = 1;
+        break;
+      }
+      const int other = nodes[node].other;
+      if (min(node, other) < lo || max(node, other) > hi) {
+        frontier[atomic_add_relaxed(frontier_count, 1u)] = child;
+        break;
+      }
+      unsigned prev = __hip_atomic_fetch_add(&s_visit[node - lo], 1u, __ATOMIC_ACQ_REL,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (prev != 1u) {  // first arrival stops; a third would mean a malformed tree
+        if (prev > 1u) *error = 1;
+        break;
+      }
+      if (!combine_node(pos_sorted, nm1, nodes_total, nodes, box_lo, box_hi, node)) {
+        *error = 1;
+        break;
+      }
+      child = node;
+      node = parent[node];
+    }
+  }
+}
+
+// Pass 2: one workgroup climbs from every frontier entry through the nodes
+// that span workgroups (pass 1's results are visible across the kernel
+// boundary; everything in here is one workgroup, so workgroup scope holds).
+__global__ __launch_bounds__(kComThreads) void com_top_kernel(
+    const float4* __restrict__ pos_sorted, int n, Node* nodes, const int* __restrict__ parent,
+    unsigned* visit, float4* box_lo, float4* box_hi, const int* __restrict__ frontier,
+    const unsigned* frontier_count, int* error) {
+  const int nm1 = n - 1;
+  const unsigned nodes_total = 2u * (unsigned)n - 1u;
+  const unsigned count = *frontier_count;
+  for (unsigned e = threadIdx.x; e < count; e += blockDim.x) {
+    const int start = frontier[e];
+    if ((unsigned)start >= nodes_total) {
       *error = 1;
-      return;
+      continue;
     }
-    // first arrival stops; exactly the second continues (a third arrival
-    // would mean a malformed tree: stop instead of looping)
-    unsigned prev = atomic_add_agent(&visit[node], 1u);
-    if (prev != 1u) {
-      if (prev > 1u) *error = 1;
-      return;
+    int node = parent[start];
+    while (node >= 0) {
+      if (node >= nm1) {
+        *error = 1;
+        break;
+      }
+      unsigned prev = atomic_add_wg(&visit[node], 1u);
+      if (prev != 1u) {
+        if (prev > 1u) *error = 1;
+        break;
+      }
+      if (!combine_node(pos_sorted, nm1, nodes_total, nodes, box_lo, box_hi, node)) {
+        *error = 1;
+        break;
+      }
+      node = parent[node];
     }
-    int2 ch = *reinterpret_cast<const int2*>(&nodes[node].left);
-    if ((unsigned)ch.x >= nodes_total || (unsigned)ch.y >= nodes_total) {
-      *error = 1;
-      return;
-    }
-    float4 cl, ll, hl, cr, lr, hr;
-    if (ch.x >= nm1) {
-      cl = pos_sorted[ch.x - nm1];
-      ll = hl = cl;
-    } else {
-      cl = nodes[ch.x].com;
-      ll = box_lo[ch.x];
-      hl = box_hi[ch.x];
-    }
-    if (ch.y >= nm1) {
-      cr = pos_sorted[ch.y - nm1];
-      lr = hr = cr;
-    } else {
-      cr = nodes[ch.y].com;
-      lr = box_lo[ch.y];
-      hr = box_hi[ch.y];
-    }
-    double ml = (double)cl.w, mr = (double)cr.w, m = ml + mr;
-    float4 c;
-    c.x = (float)(((double)cl.x * ml + (double)cr.x * mr) / m);
-    c.y = (float)(((double)cl.y * ml + (double)cr.y * mr) / m);
-    c.z = (float)(((double)cl.z * ml + (double)cr.z * mr) / m);
-    c.w = (float)m;
-    float4 lo = make_float4(fminf(ll.x, lr.x), fminf(ll.y, lr.y), fminf(ll.z, lr.z), 0.f);
-    float4 hi = make_float4(fmaxf(hl.x, hr.x), fmaxf(hl.y, hr.y), fmaxf(hl.z, hr.z), 0.f);
-    float s = fmaxf(hi.x - lo.x, fmaxf(hi.y - lo.y, hi.z - lo.z));
-    nodes[node].com = c;
-    nodes[node].size2 = s * s;
-    box_lo[node] = lo;
-    box_hi[node] = hi;
-    node = parent[node];
   }
 }
 
@@ -397,22 +468,31 @@ void launch_sort(hipStream_t s, int key_bits, void* keys_a, void* keys_b, unsign
 }
 
 void launch_karras(hipStream_t s, int key_bits, const void* keys, int n, Node* nodes, int* parent,
-                   unsigned* visit) {
+                   unsigned* visit, unsigned* frontier_count) {
   if (n < 2) return;
   int g = (n - 1 + kBlock - 1) / kBlock;
   if (key_bits == 30)
-    karras_kernel<uint32_t><<<g, kBlock, 0, s>>>((const uint32_t*)keys, n, nodes, parent, visit);
+    karras_kernel<uint32_t><<<g, kBlock, 0, s>>>((const uint32_t*)keys, n, nodes, parent, visit,
+                                                 frontier_count);
   else
-    karras_kernel<uint64_t><<<g, kBlock, 0, s>>>((const uint64_t*)keys, n, nodes, parent, visit);
+    karras_kernel<uint64_t><<<g, kBlock, 0, s>>>((const uint64_t*)keys, n, nodes, parent, visit,
+                                                 frontier_count);
   NB_LAUNCH_CHECK();
 }
 
 void launch_com(hipStream_t s, const float4* pos_sorted, int n, Node* nodes, const int* parent,
-                unsigned* visit, float4* box_lo, float4* box_hi, int* error) {
+                unsigned* visit, float4* box_lo, float4* box_hi, int* frontier,
+                unsigned* frontier_count, int* error) {
   if (n < 2) return;
-  int g = (n + kBlock - 1) / kBlock;
-  com_kernel<<<g, kBlock, 0, s>>>(pos_sorted, n, nodes, parent, visit, box_lo, box_hi, error);
+  int g = (n + kComLeaves - 1) / kComLeaves;
+  com_local_kernel<<<g, kComThreads, 0, s>>>(pos_sorted, n, nodes, parent, box_lo, box_hi,
+                                             frontier, frontier_count, error);
   NB_LAUNCH_CHECK();
+  if (g > 1) {
+    com_top_kernel<<<1, kComThreads, 0, s>>>(pos_sorted, n, nodes, parent, visit, box_lo,
+                                             box_hi, frontier, frontier_count, error);
+    NB_LAUNCH_CHECK();
+  }
 }
 
 }  // namespace nb

@@ -98,70 +98,167 @@ __global__ __launch_bounds__(kBlock) void walk_thread_kernel(
   finish(t, n, ax, ay, az, G, perm, acc, vel, half_dt);
 }
 
-template <bool COUNT>
+// Node-parallel wave64 group walk.  The visited / opened / accepted node sets
+// are exactly those of the one-node-at-a-time group vote (a node is visited
+// iff all its ancestors were opened, and each open decision depends only on
+// the node and the group), so the interaction and opening-test counts are
+// identical; only the FP32 summation order differs.
+//
+// Per iteration the wave pops up to 64 node ids from its LDS stack, lane k
+// loads node k (independent vector loads), and the open/accept vote of each
+// node over the 64 targets is settled by
+//   * a prefilter on the group's bounding box: a node is accepted by every
+//     lane if size2 < theta2 * dmin2 (dmin = COM-to-box distance), opened by
+//     every lane if size2 >= theta2 * dmax2 (farthest box corner), each with
+//     a 1e-5 relative margin so FP32 rounding cannot flip a lane's test;
+//   * otherwise the exact per-lane test in the canonical order, node by node
+//     with the node broadcast by readlane, reduced by ballot.
+// Opened nodes push their two children; accepted nodes and leaves append
+// their (x, y, z, m) to an LDS interaction list that all 64 lanes evaluate in
+// blocks of 64 (LDS broadcast reads, pure FP32 VALU), optionally Kahan-adding
+// each block's partial sums (compensated mode).
+constexpr int kWaveStack = 2048;   // LDS stack entries per wave
+constexpr int kWaveList = 128;     // LDS interaction-list entries per wave
+constexpr float kMacMargin = 1e-5f;
+
+template <bool COMP>
+__device__ __forceinline__ void flush_list(const float4* list, int cnt, float bx, float by,
+                                           float bz, float eps2, float& ax, float& ay, float& az,
+                                           float& cx, float& cy, float& cz) {
+  float px = 0.f, py = 0.f, pz = 0.f;
+  if (!COMP) { px = ax; py = ay; pz = az; }
+#pragma unroll 8
+  for (int k = 0; k < cnt; ++k) {
+    const float4 src = list[k];
+    const float dx = src.x - bx, dy = src.y - by, dz = src.z - bz;
+    accumulate(dx, dy, dz, dist2(dx, dy, dz), src.w, eps2, px, py, pz);
+  }
+  if (COMP) {
+    kahan_add(ax, cx, px);
+    kahan_add(ay, cy, py);
+    kahan_add(az, cz, pz);
+  } else {
+    ax = px; ay = py; az = pz;
+  }
+}
+
+template <bool COUNT, bool COMP>
 __global__ __launch_bounds__(kBlock) void walk_wave_kernel(
     const float4* __restrict__ pos_sorted, const unsigned* __restrict__ perm, int n,
     const Node* __restrict__ nodes, float theta2, float eps2, float G, float4* __restrict__ acc,
     float4* __restrict__ vel, float half_dt, int* __restrict__ counts, int* __restrict__ opens,
     int* overflow, int* error) {
+  constexpr int WAVES = kBlock / 64;
+  __shared__ int s_stack[WAVES][kWaveStack];
+  __shared__ float4 s_list[WAVES][kWaveList];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
   const int gbase = t - lane;
   if (gbase >= n) return;  // wave-uniform
+  int* stk = s_stack[w];
+  float4* list = s_list[w];
   const bool live = t < n;
   const int nm1 = n - 1;
   const float4 b = pos_sorted[live ? t : nm1];
-  float ax = 0.f, ay = 0.f, az = 0.f;
-  int stk0 = 0, stk1 = 0;  // stack entry k lives in lane k&63 of stk0 (k<64) or stk1
-  int sp = 1;              // entry 0 = root (0)
-  int ni = 0, no = 0;
+  const unsigned long long lt = (1ull << lane) - 1ull;
+
+  // bounding box of the live targets (dead lanes repeat the last target)
+  float blx = wave_min(b.x), bly = wave_min(b.y), blz = wave_min(b.z);
+  float bhx = wave_max(b.x), bhy = wave_max(b.y), bhz = wave_max(b.z);
+  const float acc_lo = theta2 * (1.f - kMacMargin);
+  const float acc_hi = theta2 * (1.f + kMacMargin);
+
+  float ax = 0.f, ay = 0.f, az = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
+  int sp = 1, nlist = 0;
+  long ni = 0, no = 0;
   const unsigned nodes_total = 2u * (unsigned)n - 1u;
   unsigned pops = 0;
+  if (lane == 0) stk[0] = 0;
   while (sp > 0) {
-    --sp;
-    const int id = sp < 64 ? __builtin_amdgcn_readlane(stk0, sp)
-                           : __builtin_amdgcn_readlane(stk1, sp - 64);
-    if ((unsigned)id >= nodes_total || ++pops > nodes_total) {
+    int k = sp < 64 ? sp : 64;
+    // pop k entries from the top: lane l takes stk[sp-1-l]
+    const bool valid = lane < k;
+    const int id = valid ? stk[sp - 1 - lane] : 0;
+    sp -= k;
+    pops += k;  // a walk over a valid tree pops each node at most once
+    if (__ballot(valid && (unsigned)id >= nodes_total) || pops > nodes_total) {
       if (lane == 0) *error = 1;
       break;
     }
-    float dx, dy, dz, d2, m;
-    if (id >= nm1) {
-      const float4 src = pos_sorted[id - nm1];
-      dx = src.x - b.x; dy = src.y - b.y; dz = src.z - b.z;
-      d2 = dist2(dx, dy, dz);
-      m = src.w;
-    } else {
-      const float4 c = nodes[id].com;
-      const float s2 = nodes[id].size2;
-      dx = c.x - b.x; dy = c.y - b.y; dz = c.z - b.z;
-      d2 = dist2(dx, dy, dz);
-      if (COUNT) ++no;
-      const bool open = live && !(s2 < theta2 * d2);
-      if (__ballot(open)) {
-        if (sp + 2 > kStackDepth) {
-          if (lane == 0) *overflow = 1;
-          break;
-        }
-        const int l = __builtin_amdgcn_readfirstlane(nodes[id].left);
-        const int r = __builtin_amdgcn_readfirstlane(nodes[id].right);
-        // lane (sp & 63) of the stack register takes the entry
-        if (sp < 64) stk0 = (lane == sp) ? l : stk0;
-        else stk1 = (lane == sp - 64) ? l : stk1;
-        ++sp;
-        if (sp < 64) stk0 = (lane == sp) ? r : stk0;
-        else stk1 = (lane == sp - 64) ? r : stk1;
-        ++sp;
-        continue;
-      }
-      m = c.w;
+    const bool is_leaf = valid && id >= nm1;
+    const bool is_node = valid && id < nm1;
+    float4 src = make_float4(0.f, 0.f, 0.f, 0.f);
+    float s2 = 0.f;
+    int left = 0, right = 0;
+    if (is_leaf) {
+      src = pos_sorted[id - nm1];
+    } else if (is_node) {
+      const Node nd = nodes[id];
+      src = nd.com;
+      s2 = nd.size2;
+      left = nd.left;
+      right = nd.right;
     }
-    accumulate(dx, dy, dz, d2, m, eps2, ax, ay, az);
-    if (COUNT) ++ni;
+    bool sure_open = false, ambiguous = false;
+    if (is_node) {
+      const float ex = fmaxf(fmaxf(blx - src.x, src.x - bhx), 0.f);
+      const float ey = fmaxf(fmaxf(bly - src.y, src.y - bhy), 0.f);
+      const float ez = fmaxf(fmaxf(blz - src.z, src.z - bhz), 0.f);
+      const float dmin2 = ex * ex + ey * ey + ez * ez;
+      const float fx = fmaxf(fabsf(src.x - blx), fabsf(src.x - bhx));
+      const float fy = fmaxf(fabsf(src.y - bly), fabsf(src.y - bhy));
+      const float fz = fmaxf(fabsf(src.z - blz), fabsf(src.z - bhz));
+      const float dmax2 = fx * fx + fy * fy + fz * fz;
+      const bool sure_accept = s2 < acc_lo * dmin2;
+      sure_open = s2 > acc_hi * dmax2;
+      ambiguous = !sure_accept && !sure_open;
+    }
+    unsigned long long open_mask = __ballot(sure_open);
+    unsigned long long amb = __ballot(ambiguous);
+    // exact vote for the nodes the prefilter could not settle
+    while (amb) {
+      const int q = __builtin_ctzll(amb);
+      amb &= amb - 1ull;
+      const float nx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(src.x), q));
+      const float ny = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(src.y), q));
+      const float nz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(src.z), q));
+      const float ns2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(s2), q));
+      const float dx = nx - b.x, dy = ny - b.y, dz = nz - b.z;
+      const bool op = live && !(ns2 < theta2 * dist2(dx, dy, dz));
+      if (__ballot(op)) open_mask |= 1ull << q;
+    }
+    const unsigned long long node_mask = __ballot(is_node);
+    const unsigned long long src_mask = (node_mask & ~open_mask) | __ballot(is_leaf);
+    if (COUNT) {
+      ni += __popcll(src_mask);
+      no += __popcll(node_mask);
+    }
+    // push the children of opened nodes (lane order)
+    const int nopen = __popcll(open_mask);
+    if (sp + 2 * nopen > kWaveStack) {
+      if (lane == 0) *overflow = 1;
+      break;
+    }
+    if ((open_mask >> lane) & 1ull) {
+      const int slot = sp + 2 * __popcll(open_mask & lt);
+      stk[slot] = left;
+      stk[slot + 1] = right;
+    }
+    sp += 2 * nopen;
+    // append sources to the interaction list (lane order)
+    if ((src_mask >> lane) & 1ull) list[nlist + __popcll(src_mask & lt)] = src;
+    nlist += __popcll(src_mask);
+    if (nlist >= 64) {
+      flush_list<COMP>(list, 64, b.x, b.y, b.z, eps2, ax, ay, az, cx, cy, cz);
+      nlist -= 64;
+      if (lane < nlist) list[lane] = list[64 + lane];
+    }
   }
+  if (nlist > 0) flush_list<COMP>(list, nlist, b.x, b.y, b.z, eps2, ax, ay, az, cx, cy, cz);
   if (COUNT && lane == 0) {
-    counts[gbase >> 6] = ni;
-    opens[gbase >> 6] = no;
+    counts[gbase >> 6] = (int)ni;
+    opens[gbase >> 6] = (int)no;
   }
   if (live) finish(t, n, ax, ay, az, G, perm, acc, vel, half_dt);
 }
@@ -243,7 +340,10 @@ void walk_dispatch(hipStream_t s, int traverse, int width, const WalkArgs& a) {
   if (traverse == 0) {
     walk_thread_kernel<COUNT><<<g, kBlock, 0, s>>>(NB_WALK_ARGS);
   } else if (width == 64) {
-    walk_wave_kernel<COUNT><<<g, kBlock, 0, s>>>(NB_WALK_ARGS);
+    if (a.compensated)
+      walk_wave_kernel<COUNT, true><<<g, kBlock, 0, s>>>(NB_WALK_ARGS);
+    else
+      walk_wave_kernel<COUNT, false><<<g, kBlock, 0, s>>>(NB_WALK_ARGS);
   } else if (width == 32) {
     walk_group_kernel<32, COUNT><<<g, kBlock, 0, s>>>(NB_WALK_ARGS);
   } else if (width == 16) {

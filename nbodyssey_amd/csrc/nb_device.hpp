@@ -70,6 +70,9 @@ __device__ __forceinline__ unsigned wave_incl_scan(unsigned v) {
 __device__ __forceinline__ unsigned atomic_add_agent(unsigned* p, unsigned v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ unsigned atomic_add_wg(unsigned* p, unsigned v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 __device__ __forceinline__ unsigned atomic_add_relaxed(unsigned* p, unsigned v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }

@@ -53,13 +53,14 @@ constexpr int kSortTile = kBlock * kSortItems;
 constexpr int kMaxPasses = 8;
 
 // One internal tree node, 32 bytes so a wave-uniform walk fetches it with a
-// single scalar load.  com = (x, y, z, mass); size2 = (longest AABB edge)^2.
+// single scalar load.  com = (x, y, z, mass); size2 = (longest AABB edge)^2;
+// internal node i covers the sorted leaves between i and `other` (inclusive).
 struct alignas(32) Node {
   float4 com;
   int left;
   int right;
   float size2;
-  int pad;
+  int other;
 };
 
 // Morton quantisation constants, formed in FP64 by the bbox reduction.
@@ -79,7 +80,8 @@ struct SortState {
   unsigned bbox_ticket;
   int overflow;             // traversal stack overflow flag
   int error;                // consistency guard tripped (bad sort/tree data)
-  int pad[3];
+  unsigned frontier_count;  // COM: children of workgroup-spanning nodes
+  int pad[2];
 };
 
 // ---------------------------------------------------------------- launchers
@@ -119,9 +121,10 @@ void launch_sort(hipStream_t s, int key_bits, void* keys_a, void* keys_b, unsign
 // (the sort sets SortState::error if a tile id or destination is out of range)
 int sort_passes(int key_bits);
 void launch_karras(hipStream_t s, int key_bits, const void* keys, int n, Node* nodes, int* parent,
-                   unsigned* visit);
+                   unsigned* visit, unsigned* frontier_count);
 void launch_com(hipStream_t s, const float4* pos_sorted, int n, Node* nodes, const int* parent,
-                unsigned* visit, float4* box_lo, float4* box_hi, int* error);
+                unsigned* visit, float4* box_lo, float4* box_hi, int* frontier,
+                unsigned* frontier_count, int* error);
 
 // k_traverse.hip
 struct WalkArgs {
@@ -137,6 +140,7 @@ struct WalkArgs {
   int* opens;         // same, opening tests
   int* overflow;      // set to 1 if a stack would overflow
   int* error;         // set to 1 if a node id is out of range or the walk is too long
+  bool compensated;   // Kahan-add each block of interactions (wave64 walk)
 };
 void launch_walk(hipStream_t s, int traverse, int group_width, const WalkArgs& a);
 
