@@ -66,6 +66,35 @@ def cpu_baseline(n_target, sample_n, eps, dt):
                       f"{(n_target / sample_n) ** 2:.0f} to {n_target} particles (O(n^2))"}
 
 
+def naive_main(args, sim, eps):
+    """All-pairs leapfrog steps; the step time is dominated by the all-pairs
+    kernel (the fused kick/drift add ~48 B/particle of HBM traffic)."""
+    sim.compute_forces()
+    sim.step(args.warmup)
+    sim.synchronize()
+    sim.reset_force_timing()
+    ms = sim.time_steps(args.steps)
+    force_ms = sim.force_ms_total() / max(sim.force_calls(), 1)
+    n = args.n
+    flops = 20.0 * n * n
+    achieved = flops / (force_ms * 1e-3) / 1e12
+    out = {
+        "metric": "particle-steps/s (all-pairs)", "value": n * args.steps / (ms * 1e-3),
+        "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic two-disk encounter",
+        "config": {"workload": f"two-disk {n} particles, tiled all-pairs, eps={eps}", "n": n},
+        "ginteractions_per_s": n * n / (force_ms * 1e-3) / 1e9,
+        "roofline": {"bound": "valu", "kernel": "allpairs", "achieved": achieved,
+                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "flops_per_launch": flops, "force_ms": force_ms},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    sim.close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +102,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--theta", type=float, default=0.5)
+    ap.add_argument("--force", default="bh", choices=["bh", "naive"],
+                    help="naive: time the tiled all-pairs step instead (roofline = 20 flop "
+                         "per interaction, n^2 interactions per launch)")
     ap.add_argument("--traverse", default="warp", choices=["warp", "thread"])
     ap.add_argument("--group-width", type=int, default=64)
     ap.add_argument("--key-bits", type=int, default=30)
@@ -93,9 +125,11 @@ def main():
     t_ic = time.perf_counter() - t_ic
     print(f"bench: n={args.n} ic {t_ic:.1f}s", file=sys.stderr, flush=True)
 
-    sim = engine.Simulation(pos, vel, dt=dt, eps=eps, theta=args.theta, force="bh",
+    sim = engine.Simulation(pos, vel, dt=dt, eps=eps, theta=args.theta, force=args.force,
                             traverse=args.traverse, group_width=args.group_width,
                             key_bits=args.key_bits)
+    if args.force == "naive":
+        return naive_main(args, sim, eps)
     sim.compute_forces()              # a(x0)
     sim.step(args.warmup)
     sim.synchronize()

@@ -58,10 +58,8 @@ __global__ __launch_bounds__(kBlock) void allpairs_kernel(const float4* __restri
     for (int k = 0; k < kBlock; ++k) {
       float4 src = tile[k];
 #pragma unroll
-      for (int q = 0; q < kTargets; ++q) {
-        float dx = src.x - bx[q], dy = src.y - by[q], dz = src.z - bz[q];
-        accumulate(dx, dy, dz, dist2(dx, dy, dz), src.w, eps2, px[q], py[q], pz[q]);
-      }
+      for (int q = 0; q < kTargets; ++q)
+        accumulate_src(src, bx[q], by[q], bz[q], eps2, px[q], py[q], pz[q]);
     }
 #pragma unroll
     for (int q = 0; q < kTargets; ++q) {

@@ -120,6 +120,7 @@ __global__ __launch_bounds__(kBlock) void walk_thread_kernel(
 constexpr int kWaveStack = 2048;   // LDS stack entries per wave
 constexpr int kWaveList = 128;     // LDS interaction-list entries per wave
 constexpr float kMacMargin = 1e-5f;
+constexpr int kReps = 8;           // representative targets for the open shortcut
 
 template <bool COMP>
 __device__ __forceinline__ void flush_list(const float4* list, int cnt, float bx, float by,
@@ -128,11 +129,7 @@ __device__ __forceinline__ void flush_list(const float4* list, int cnt, float bx
   float px = 0.f, py = 0.f, pz = 0.f;
   if (!COMP) { px = ax; py = ay; pz = az; }
 #pragma unroll 8
-  for (int k = 0; k < cnt; ++k) {
-    const float4 src = list[k];
-    const float dx = src.x - bx, dy = src.y - by, dz = src.z - bz;
-    accumulate(dx, dy, dz, dist2(dx, dy, dz), src.w, eps2, px, py, pz);
-  }
+  for (int k = 0; k < cnt; ++k) accumulate_src(list[k], bx, by, bz, eps2, px, py, pz);
   if (COMP) {
     kahan_add(ax, cx, px);
     kahan_add(ay, cy, py);
@@ -168,6 +165,18 @@ __global__ __launch_bounds__(kBlock) void walk_wave_kernel(
   float bhx = wave_max(b.x), bhy = wave_max(b.y), bhz = wave_max(b.z);
   const float acc_lo = theta2 * (1.f - kMacMargin);
   const float acc_hi = theta2 * (1.f + kMacMargin);
+  // representative targets spread over the group (Morton order spreads them
+  // in space); if one of them opens a node by its own canonical test, the
+  // node is open -- an exact shortcut, no margin needed.  A dead lane holds
+  // pos_sorted[n-1], a live member of this (last) group, so it is valid too.
+  float rx[kReps], ry[kReps], rz[kReps];
+#pragma unroll
+  for (int r = 0; r < kReps; ++r) {
+    const int src_lane = r * 63 / (kReps - 1);
+    rx[r] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b.x), src_lane));
+    ry[r] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b.y), src_lane));
+    rz[r] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b.z), src_lane));
+  }
 
   float ax = 0.f, ay = 0.f, az = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
   int sp = 1, nlist = 0;
@@ -212,6 +221,11 @@ __global__ __launch_bounds__(kBlock) void walk_wave_kernel(
       const float dmax2 = fx * fx + fy * fy + fz * fz;
       const bool sure_accept = s2 < acc_lo * dmin2;
       sure_open = s2 > acc_hi * dmax2;
+#pragma unroll
+      for (int r = 0; r < kReps; ++r) {
+        const float dx = src.x - rx[r], dy = src.y - ry[r], dz = src.z - rz[r];
+        sure_open = sure_open || !(s2 < theta2 * dist2(dx, dy, dz));
+      }
       ambiguous = !sure_accept && !sure_open;
     }
     unsigned long long open_mask = __ballot(sure_open);

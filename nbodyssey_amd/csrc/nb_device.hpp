@@ -25,6 +25,19 @@ __device__ __forceinline__ void accumulate(float dx, float dy, float dz, float d
   az = fmaf(dz, f, az);
 }
 
+// Same law for a source with no precomputed d2: the softening is folded
+// into the first FMA of |d|^2 (one instruction fewer per interaction).
+__device__ __forceinline__ void accumulate_src(float4 src, float bx, float by, float bz,
+                                               float eps2, float& ax, float& ay, float& az) {
+  const float dx = src.x - bx, dy = src.y - by, dz = src.z - bz;
+  const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
+  const float inv = rsq(r2);
+  const float f = src.w * inv * inv * inv;
+  ax = fmaf(dx, f, ax);
+  ay = fmaf(dy, f, ay);
+  az = fmaf(dz, f, az);
+}
+
 // Kahan-compensated add of `x` into (sum, c).
 __device__ __forceinline__ void kahan_add(float& sum, float& c, float x) {
   float y = x - c;
