@@ -66,6 +66,20 @@ def cpu_baseline(n_target, sample_n, eps, dt):
                       f"{(n_target / sample_n) ** 2:.0f} to {n_target} particles (O(n^2))"}
 
 
+def profiled_walk_traffic(args):
+    """HBM bytes per walk launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (calibrated against plain streaming kernels), for the
+    default workload only; None otherwise."""
+    default = (args.n == 1 << 20 and args.theta == 0.5 and args.traverse == "warp"
+               and args.group_width == 64 and args.key_bits == 30)
+    path = os.path.join(ROOT, "profiles", "bh_step_1m_theta05_traffic.md")
+    if not default or not os.path.exists(path):
+        return None
+    import re
+    m = re.search(r"traffic_bytes_per_launch = (\d+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
 def naive_main(args, sim, eps):
     """All-pairs leapfrog steps; the step time is dominated by the all-pairs
     kernel (the fused kick/drift add ~48 B/particle of HBM traffic)."""
@@ -198,7 +212,8 @@ def main():
         "phase_ms": {k: phase[i] / max(calls, 1) for i, k in enumerate(names)},
         "roofline": {"bound": "valu", "kernel": "walk",
                      "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": profiled_walk_traffic(args),
+                     "traffic_unit": "bytes/launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, calibrated)",
                      "flops_per_launch": flops, "walk_ms": walk_ms,
                      "per_target_interactions": float(inter.mean()),
                      "per_target_opens": float(opens.mean()),
