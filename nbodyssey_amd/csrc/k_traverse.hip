@@ -117,7 +117,8 @@ __global__ __launch_bounds__(kBlock) void walk_thread_kernel(
 // their (x, y, z, m) to an LDS interaction list that all 64 lanes evaluate in
 // blocks of 64 (LDS broadcast reads, pure FP32 VALU), optionally Kahan-adding
 // each block's partial sums (compensated mode).
-constexpr int kWaveStack = 2048;   // LDS stack entries per wave
+constexpr int kWaveStack = 1024;   // LDS stack entries per wave
+constexpr int kStackReserve = 64 + 2 * kStackDepth;
 constexpr int kWaveList = 128;     // LDS interaction-list entries per wave
 constexpr float kMacMargin = 1e-5f;
 constexpr int kReps = 8;           // representative targets for the open shortcut
@@ -185,7 +186,11 @@ __global__ __launch_bounds__(kBlock) void walk_wave_kernel(
   unsigned pops = 0;
   if (lane == 0) stk[0] = 0;
   while (sp > 0) {
+    // pop up to 64 entries; near the top of the stack fall back to one at a
+    // time (a depth-first walk needs at most one entry per tree level, and
+    // the reserve exceeds the deepest tree), so the stack cannot overflow
     int k = sp < 64 ? sp : 64;
+    if (sp > kWaveStack - kStackReserve) k = 1;
     // pop k entries from the top: lane l takes stk[sp-1-l]
     const bool valid = lane < k;
     const int id = valid ? stk[sp - 1 - lane] : 0;
@@ -220,27 +225,35 @@ __global__ __launch_bounds__(kBlock) void walk_wave_kernel(
       const float fz = fmaxf(fabsf(src.z - blz), fabsf(src.z - bhz));
       const float dmax2 = fx * fx + fy * fy + fz * fz;
       const bool sure_accept = s2 < acc_lo * dmin2;
-      sure_open = s2 > acc_hi * dmax2;
+      // all representative tests are evaluated (no short-circuit branches):
+      // open if one of them has theta2*d2 <= s2, i.e. the smallest d2 decides
+      float rmin = dist2(src.x - rx[0], src.y - ry[0], src.z - rz[0]);
 #pragma unroll
-      for (int r = 0; r < kReps; ++r) {
-        const float dx = src.x - rx[r], dy = src.y - ry[r], dz = src.z - rz[r];
-        sure_open = sure_open || !(s2 < theta2 * dist2(dx, dy, dz));
-      }
-      ambiguous = !sure_accept && !sure_open;
+      for (int r = 1; r < kReps; ++r)
+        rmin = fminf(rmin, dist2(src.x - rx[r], src.y - ry[r], src.z - rz[r]));
+      sure_open = (s2 > acc_hi * dmax2) | !(s2 < theta2 * rmin);
+      ambiguous = !sure_accept & !sure_open;
     }
     unsigned long long open_mask = __ballot(sure_open);
     unsigned long long amb = __ballot(ambiguous);
-    // exact vote for the nodes the prefilter could not settle
+    // exact vote for the nodes the prefilter could not settle, two at a time
     while (amb) {
-      const int q = __builtin_ctzll(amb);
+      const int q0 = __builtin_ctzll(amb);
       amb &= amb - 1ull;
-      const float nx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(src.x), q));
-      const float ny = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(src.y), q));
-      const float nz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(src.z), q));
-      const float ns2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(s2), q));
-      const float dx = nx - b.x, dy = ny - b.y, dz = nz - b.z;
-      const bool op = live && !(ns2 < theta2 * dist2(dx, dy, dz));
-      if (__ballot(op)) open_mask |= 1ull << q;
+      const int q1 = amb ? __builtin_ctzll(amb) : q0;
+      amb &= amb - 1ull;
+      const float x0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(src.x), q0));
+      const float y0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(src.y), q0));
+      const float z0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(src.z), q0));
+      const float t0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(s2), q0));
+      const float x1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(src.x), q1));
+      const float y1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(src.y), q1));
+      const float z1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(src.z), q1));
+      const float t1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(s2), q1));
+      const bool op0 = live & !(t0 < theta2 * dist2(x0 - b.x, y0 - b.y, z0 - b.z));
+      const bool op1 = live & !(t1 < theta2 * dist2(x1 - b.x, y1 - b.y, z1 - b.z));
+      if (__ballot(op0)) open_mask |= 1ull << q0;
+      if (__ballot(op1)) open_mask |= 1ull << q1;
     }
     const unsigned long long node_mask = __ballot(is_node);
     const unsigned long long src_mask = (node_mask & ~open_mask) | __ballot(is_leaf);
