@@ -142,11 +142,10 @@ class Engine {
       launch_sort(stream, p.key_bits, keys_a.p, keys_b.p, vals_a.p, vals_b.p, n, state.p,
                   lookback.p, max_tiles, true, pos, pos_sorted.p);
       mark(ce, kPhSort + 1);
-      launch_karras(stream, p.key_bits, keys_a.p, n, nodes.p, parent.p, visit.p,
-                    &state.p->frontier_count);
+      launch_karras(stream, p.key_bits, keys_a.p, n, nodes.p, up.p, visit.p);
       mark(ce, kPhBuild + 1);
-      launch_com(stream, pos_sorted.p, n, nodes.p, parent.p, visit.p, box_lo.p, box_hi.p,
-                 frontier.p, &state.p->frontier_count, &state.p->error);
+      launch_com(stream, pos_sorted.p, n, nodes.p, up.p, visit.p, box_lo.p, box_hi.p,
+                 frontier.p, &state.p->error);
       mark(ce, kPhCom + 1);
       WalkArgs w{};
       w.pos_sorted = pos_sorted.p;
@@ -231,11 +230,11 @@ class Engine {
     vals_b.alloc(c);
     pos_sorted.alloc(c);
     nodes.alloc(c);
-    parent.alloc(2 * c);
+    up.alloc(2 * c);
     visit.alloc(c);
     box_lo.alloc(c);
     box_hi.alloc(c);
-    frontier.alloc(c + 1);
+    frontier.alloc((size_t)com_scratch_ints(n));
     bh_partial.alloc(2 * kBboxBlocks);
     quant.alloc(1, true);
     state.alloc(1, true);
@@ -245,7 +244,7 @@ class Engine {
   }
   void release_bh() {
     keys_a.reset(); keys_b.reset(); vals_a.reset(); vals_b.reset(); pos_sorted.reset();
-    nodes.reset(); parent.reset(); visit.reset(); box_lo.reset(); box_hi.reset(); frontier.reset();
+    nodes.reset(); up.reset(); visit.reset(); box_lo.reset(); box_hi.reset(); frontier.reset();
     bh_partial.reset(); quant.reset(); state.reset(); lookback.reset();
     bh_cap = 0;
     tree_valid = false;
@@ -332,7 +331,8 @@ class Engine {
   DevBuf<unsigned> vals_a, vals_b, visit;
   DevBuf<float4> pos_sorted, box_lo, box_hi, bh_partial;
   DevBuf<Node> nodes;
-  DevBuf<int> parent, frontier;
+  DevBuf<int> frontier;
+  DevBuf<int4> up;
   DevBuf<QuantParams> quant;
   DevBuf<SortState> state;
   DevBuf<unsigned long long> lookback;
@@ -619,8 +619,11 @@ int nb_get_tree(nb_engine* h, uint64_t* keys_sorted, uint32_t* perm, int32_t* ch
       }
     }
     if (perm) NB_HIP(hipMemcpy(perm, e.vals_a.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    if (parent)
-      NB_HIP(hipMemcpy(parent, e.parent.p, (2 * (size_t)n - 1) * sizeof(int), hipMemcpyDeviceToHost));
+    if (parent) {
+      std::vector<int4> u(2 * (size_t)n - 1);
+      NB_HIP(hipMemcpy(u.data(), e.up.p, u.size() * sizeof(int4), hipMemcpyDeviceToHost));
+      for (size_t k = 0; k < u.size(); ++k) parent[k] = u[k].x;
+    }
     if (pos_sorted4)
       NB_HIP(hipMemcpy(pos_sorted4, e.pos_sorted.p, n * sizeof(float4), hipMemcpyDeviceToHost));
     if (children || com4 || size2) {
@@ -763,7 +766,7 @@ int nb_karras_build(const uint64_t* keys_sorted, int n, int key_bits, int32_t* c
     nb::check_device();
     nb::DevBuf<uint64_t> dk;
     nb::DevBuf<nb::Node> nodes;
-    nb::DevBuf<int> par;
+    nb::DevBuf<int4> par;
     nb::DevBuf<unsigned> visit;
     dk.alloc(n);
     if (key_bits == 30) {
@@ -776,10 +779,13 @@ int nb_karras_build(const uint64_t* keys_sorted, int n, int key_bits, int32_t* c
     nodes.alloc(n - 1);
     par.alloc(2 * (size_t)n - 1);
     visit.alloc(n - 1);
-    nb::launch_karras(nullptr, key_bits, dk.p, n, nodes.p, par.p, visit.p, nullptr);
+    nb::launch_karras(nullptr, key_bits, dk.p, n, nodes.p, par.p, visit.p);
     NB_HIP(hipDeviceSynchronize());
-    if (parent)
-      NB_HIP(hipMemcpy(parent, par.p, (2 * (size_t)n - 1) * sizeof(int), hipMemcpyDeviceToHost));
+    if (parent) {
+      std::vector<int4> u(2 * (size_t)n - 1);
+      NB_HIP(hipMemcpy(u.data(), par.p, u.size() * sizeof(int4), hipMemcpyDeviceToHost));
+      for (size_t k = 0; k < u.size(); ++k) parent[k] = u[k].x;
+    }
     if (children) {
       std::vector<nb::Node> nd(n - 1);
       NB_HIP(hipMemcpy(nd.data(), nodes.p, (n - 1) * sizeof(nb::Node), hipMemcpyDeviceToHost));

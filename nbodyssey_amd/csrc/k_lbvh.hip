@@ -19,8 +19,9 @@
 // Karras build (forces_bh.cu:234-277): one thread per internal node.
 // COM (forces_bh.cu:281-331): leaves climb towards the root and the second
 // arrival at a node combines both children.  Pass 1 does this per workgroup
-// over 4096 consecutive leaves with LDS arrival counters; pass 2 (one
-// workgroup) finishes the few nodes whose leaf range spans workgroups.  No
+// over 1024 consecutive leaves with LDS arrival counters, pass 2 per 65,536
+// leaves with 2-bit LDS counters, and pass 3 (one workgroup) finishes the few
+// nodes whose leaf range spans superblocks.  No
 // step needs device-wide release/acquire (which would write back / invalidate
 // an XCD's L2 on every climb step).  The combine is FP64 from FP32 children,
 // rounded once.
@@ -230,11 +231,9 @@ __device__ __forceinline__ int delta(const K* __restrict__ keys, int n, int i, K
 template <typename K>
 __global__ __launch_bounds__(kBlock) void karras_kernel(const K* __restrict__ keys, int n,
                                                         Node* __restrict__ nodes,
-                                                        int* __restrict__ parent,
-                                                        unsigned* __restrict__ visit,
-                                                        unsigned* __restrict__ frontier_count) {
+                                                        int4* __restrict__ up,
+                                                        unsigned* __restrict__ visit) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0 && frontier_count) *frontier_count = 0;
   if (i >= n - 1) return;
   const K ki = keys[i];
   int d = (delta(keys, n, i, ki, i + 1) - delta(keys, n, i, ki, i - 1)) >= 0 ? 1 : -1;
@@ -257,134 +256,279 @@ __global__ __launch_bounds__(kBlock) void karras_kernel(const K* __restrict__ ke
   int right = (split + 1 == last) ? (n - 1 + split + 1) : split + 1;
   *reinterpret_cast<int2*>(&nodes[i].left) = make_int2(left, right);
   nodes[i].other = j;
-  parent[left] = i;
-  parent[right] = i;
+  // each child's upward record: parent, sibling, far end of the parent's range
+  up[left] = make_int4(i, right, j, 0);
+  up[right] = make_int4(i, left, j, 0);
   visit[i] = 0;
-  if (i == 0) parent[0] = -1;
+  if (i == 0) up[0] = make_int4(-1, -1, j, 0);
 }
 
-// Combine the two children of internal node `node` (forces_bh.cu:293-324):
-// FP64 mass-weighted centre of mass rounded once to FP32, AABB union, squared
-// longest edge.  Returns false if a child id is out of range.
-__device__ __forceinline__ bool combine_node(const float4* __restrict__ pos_sorted, int nm1,
-                                             unsigned nodes_total, Node* nodes, float4* box_lo,
-                                             float4* box_hi, int node) {
-  int2 ch = *reinterpret_cast<const int2*>(&nodes[node].left);
-  if ((unsigned)ch.x >= nodes_total || (unsigned)ch.y >= nodes_total) return false;
-  float4 cl, ll, hl, cr, lr, hr;
-  if (ch.x >= nm1) {
-    cl = pos_sorted[ch.x - nm1];
-    ll = hl = cl;
-  } else {
-    cl = nodes[ch.x].com;
-    ll = box_lo[ch.x];
-    hl = box_hi[ch.x];
-  }
-  if (ch.y >= nm1) {
-    cr = pos_sorted[ch.y - nm1];
-    lr = hr = cr;
-  } else {
-    cr = nodes[ch.y].com;
-    lr = box_lo[ch.y];
-    hr = box_hi[ch.y];
-  }
-  double ml = (double)cl.w, mr = (double)cr.w, m = ml + mr;
-  float4 c;
-  c.x = (float)(((double)cl.x * ml + (double)cr.x * mr) / m);
-  c.y = (float)(((double)cl.y * ml + (double)cr.y * mr) / m);
-  c.z = (float)(((double)cl.z * ml + (double)cr.z * mr) / m);
-  c.w = (float)m;
-  float4 lo = make_float4(fminf(ll.x, lr.x), fminf(ll.y, lr.y), fminf(ll.z, lr.z), 0.f);
-  float4 hi = make_float4(fmaxf(hl.x, hr.x), fmaxf(hl.y, hr.y), fmaxf(hl.z, hr.z), 0.f);
-  float s = fmaxf(hi.x - lo.x, fmaxf(hi.y - lo.y, hi.z - lo.z));
-  nodes[node].com = c;
-  nodes[node].size2 = s * s;
-  box_lo[node] = lo;
-  box_hi[node] = hi;
-  return true;
+// Node data a climbing thread carries: centre of mass (x, y, z, m) and AABB.
+struct NodeData {
+  float4 com, lo, hi;
+};
+
+__device__ __forceinline__ NodeData leaf_data(const float4* __restrict__ pos_sorted, int leaf) {
+  const float4 p = pos_sorted[leaf];
+  return NodeData{p, p, p};
 }
 
-// Pass 1: workgroup b owns the sorted leaves [b*L, b*L + L).  Internal node i
-// covers the leaves between i and nodes[i].other, so a node whose range lies
-// inside the workgroup's leaves has its id in [lo, hi] and is reached only by
-// this workgroup: its arrival counter lives in LDS.  A climb that reaches a
-// node spanning workgroups stops and records the finished child in
-// `frontier` for pass 2.
+__device__ __forceinline__ NodeData load_data(const float4* __restrict__ pos_sorted, int nm1,
+                                              const Node* nodes, const float4* box_lo,
+                                              const float4* box_hi, int id) {
+  if (id >= nm1) return leaf_data(pos_sorted, id - nm1);
+  return NodeData{nodes[id].com, box_lo[id], box_hi[id]};
+}
+
+// Combine two children (forces_bh.cu:313-324): FP64 mass-weighted centre of
+// mass rounded once to FP32, AABB union, squared longest edge.  The formula is
+// symmetric in the two children (IEEE + and fmin/fmax commute), so the caller
+// may pass them in either order.
+__device__ __forceinline__ NodeData combine(const NodeData& a, const NodeData& b) {
+  const double ma = (double)a.com.w, mb = (double)b.com.w, m = ma + mb;
+  NodeData r;
+  r.com.x = (float)(((double)a.com.x * ma + (double)b.com.x * mb) / m);
+  r.com.y = (float)(((double)a.com.y * ma + (double)b.com.y * mb) / m);
+  r.com.z = (float)(((double)a.com.z * ma + (double)b.com.z * mb) / m);
+  r.com.w = (float)m;
+  r.lo = make_float4(fminf(a.lo.x, b.lo.x), fminf(a.lo.y, b.lo.y), fminf(a.lo.z, b.lo.z), 0.f);
+  r.hi = make_float4(fmaxf(a.hi.x, b.hi.x), fmaxf(a.hi.y, b.hi.y), fmaxf(a.hi.z, b.hi.z), 0.f);
+  return r;
+}
+
+__device__ __forceinline__ void store_node(Node* nodes, float4* box_lo, float4* box_hi, int id,
+                                           const NodeData& d) {
+  const float s = fmaxf(d.hi.x - d.lo.x, fmaxf(d.hi.y - d.lo.y, d.hi.z - d.lo.z));
+  nodes[id].com = d.com;
+  nodes[id].size2 = s * s;
+  box_lo[id] = d.lo;
+  box_hi[id] = d.hi;
+}
+
+// Frontier lists between the passes live in fixed per-workgroup segments
+// (a contiguous leaf range splits into at most ~2 x depth maximal subtrees, far
+// below the segment size; overflow is reported as an error), so the passes
+// need no global atomics: entries are collected in LDS and one count per
+// workgroup is written.
 constexpr int kComThreads = 1024;
-constexpr int kComLeaves = 4096;
+constexpr int kComLeaves = 1024;    // pass-1 leaves per workgroup (one per thread)
+constexpr int kComSuper = 65536;    // pass-2 leaves per workgroup
+constexpr int kSeg = 256;           // frontier entries per workgroup segment
+constexpr int kTopSegs = 4096;      // superblocks the top pass can take (n <= 2^28)
 
+__device__ __forceinline__ void seg_push(int* s_list, unsigned* s_cnt, int v, int* error) {
+  const unsigned k = __hip_atomic_fetch_add(s_cnt, 1u, __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_WORKGROUP);
+  if (k < (unsigned)kSeg) s_list[k] = v;
+  else *error = 1;
+}
+
+// Pass 1: workgroup b owns the sorted leaves [b*L, b*L + L), one per thread.
+// Internal node i covers the leaves between i and its `other` end, so a node
+// whose range lies inside the workgroup's leaves has its id in [lo, hi] and is
+// reached only by this workgroup.  The workgroup preloads the upward records
+// of its leaves and of internal nodes lo..hi into LDS (coalesced), keeps the
+// arrival counters and the finished nodes' COM/AABB in LDS, and so climbs
+// without dependent global loads: global memory sees the leaf positions, the
+// final node stores and the frontier.  A climb that reaches a node spanning
+// workgroups stops and records the finished child (with its AABB, which only
+// the later passes need) in the workgroup's frontier segment.
 __global__ __launch_bounds__(kComThreads) void com_local_kernel(
-    const float4* __restrict__ pos_sorted, int n, Node* nodes, const int* __restrict__ parent,
-    float4* box_lo, float4* box_hi, int* frontier, unsigned* frontier_count, int* error) {
+    const float4* __restrict__ pos_sorted, int n, Node* nodes, const int4* __restrict__ up,
+    float4* box_lo, float4* box_hi, int* seg1, unsigned* cnt1, int* error) {
   __shared__ unsigned s_visit[kComLeaves];
-  for (int k = threadIdx.x; k < kComLeaves; k += blockDim.x) s_visit[k] = 0;
-  __syncthreads();
+  __shared__ int3 s_up_leaf[kComLeaves];
+  __shared__ int3 s_up_node[kComLeaves];
+  __shared__ float4 s_com[kComLeaves];
+  __shared__ float3 s_lo[kComLeaves], s_hi[kComLeaves];
+  __shared__ int s_front[kSeg];
+  __shared__ unsigned s_nfront;
   const int nm1 = n - 1;
-  const unsigned nodes_total = 2u * (unsigned)n - 1u;
+  const int total = 2 * n - 1;
   const int lo = blockIdx.x * kComLeaves;
   const int hi = min(n - 1, lo + kComLeaves - 1);
-  for (int leaf = lo + (int)threadIdx.x; leaf <= hi; leaf += blockDim.x) {
+  const int t = threadIdx.x;
+  s_visit[t] = 0;
+  if (t == 0) s_nfront = 0;
+  if (lo + t <= hi) {
+    const int4 ul = up[nm1 + lo + t];
+    s_up_leaf[t] = make_int3(ul.x, ul.y, ul.z);
+    if (lo + t < nm1) {
+      const int4 un = up[lo + t];
+      s_up_node[t] = make_int3(un.x, un.y, un.z);
+    }
+  }
+  __syncthreads();
+  const int leaf = lo + t;
+  if (leaf <= hi) {
     int child = nm1 + leaf;
-    int node = parent[child];
-    while (node >= 0) {
-      if (node >= nm1) {
+    NodeData cur = leaf_data(pos_sorted, leaf);
+    int3 u = s_up_leaf[t];
+    while (true) {
+      const int node = u.x;
+      if (node < 0) break;  // reached the root
+      if (node >= nm1 || (unsigned)u.y >= (unsigned)total) {
         *error = 1;
         break;
       }
-      const int other = nodes[node].other;
-      if (min(node, other) < lo || max(node, other) > hi) {
-        frontier[atomic_add_relaxed(frontier_count, 1u)] = child;
+      if (min(node, u.z) < lo || max(node, u.z) > hi) {
+        // the finished child is a frontier entry: later passes need its AABB
+        if (child < nm1) {
+          box_lo[child] = cur.lo;
+          box_hi[child] = cur.hi;
+        }
+        seg_push(s_front, &s_nfront, child, error);
         break;
       }
-      unsigned prev = __hip_atomic_fetch_add(&s_visit[node - lo], 1u, __ATOMIC_ACQ_REL,
-                                             __HIP_MEMORY_SCOPE_WORKGROUP);
+      const int slot = node - lo;
+      const unsigned prev = __hip_atomic_fetch_add(&s_visit[slot], 1u, __ATOMIC_ACQ_REL,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
       if (prev != 1u) {  // first arrival stops; a third would mean a malformed tree
         if (prev > 1u) *error = 1;
         break;
       }
-      if (!combine_node(pos_sorted, nm1, nodes_total, nodes, box_lo, box_hi, node)) {
-        *error = 1;
-        break;
+      // the sibling is a local leaf (global) or a local internal node (LDS)
+      NodeData sib;
+      if (u.y >= nm1) {
+        sib = leaf_data(pos_sorted, u.y - nm1);
+      } else {
+        const int ss = u.y - lo;
+        sib.com = s_com[ss];
+        sib.lo = make_float4(s_lo[ss].x, s_lo[ss].y, s_lo[ss].z, 0.f);
+        sib.hi = make_float4(s_hi[ss].x, s_hi[ss].y, s_hi[ss].z, 0.f);
       }
+      cur = combine(cur, sib);
+      s_com[slot] = cur.com;
+      s_lo[slot] = make_float3(cur.lo.x, cur.lo.y, cur.lo.z);
+      s_hi[slot] = make_float3(cur.hi.x, cur.hi.y, cur.hi.z);
+      const float sz = fmaxf(cur.hi.x - cur.lo.x, fmaxf(cur.hi.y - cur.lo.y, cur.hi.z - cur.lo.z));
+      nodes[node].com = cur.com;
+      nodes[node].size2 = sz * sz;
       child = node;
-      node = parent[node];
+      u = s_up_node[slot];
     }
+  }
+  __syncthreads();
+  const unsigned cnt = min(s_nfront, (unsigned)kSeg);
+  if (t < (int)cnt) seg1[blockIdx.x * kSeg + t] = s_front[t];
+  if (t == 0) cnt1[blockIdx.x] = cnt;
+}
+
+// Climb through global node data with arrival counters in LDS, shared by
+// passes 2 and 3.  `counter(node)` returns the previous arrival count.
+template <typename Counter>
+__device__ __forceinline__ void climb_global(const float4* __restrict__ pos_sorted, int n,
+                                             Node* nodes, const int4* __restrict__ up,
+                                             float4* box_lo, float4* box_hi, int child,
+                                             int lo, int hi, int* s_list, unsigned* s_cnt,
+                                             Counter counter, int* error) {
+  const int nm1 = n - 1;
+  const int total = 2 * n - 1;
+  NodeData cur = load_data(pos_sorted, nm1, nodes, box_lo, box_hi, child);
+  while (true) {
+    const int4 u = up[child];
+    const int node = u.x;
+    if (node < 0) break;
+    if (node >= nm1 || (unsigned)u.y >= (unsigned)total) {
+      *error = 1;
+      break;
+    }
+    if (min(node, u.z) < lo || max(node, u.z) > hi) {
+      seg_push(s_list, s_cnt, child, error);
+      break;
+    }
+    const unsigned prev = counter(node);
+    if (prev != 1u) {
+      if (prev > 1u) *error = 1;
+      break;
+    }
+    cur = combine(cur, load_data(pos_sorted, nm1, nodes, box_lo, box_hi, u.y));
+    store_node(nodes, box_lo, box_hi, node, cur);
+    child = node;
   }
 }
 
-// Pass 2: one workgroup climbs from every frontier entry through the nodes
-// that span workgroups (pass 1's results are visible across the kernel
-// boundary; everything in here is one workgroup, so workgroup scope holds).
+// Pass 2: workgroup s owns the superblock of leaves [s*S, s*S + S): it climbs
+// from the frontier segments of its pass-1 workgroups with 2-bit arrival
+// counters in LDS (a node arrives at most twice, so adding 1 << 2k never
+// carries into the next field), indexed by node - lo.
+__global__ __launch_bounds__(kComThreads) void com_mid_kernel(
+    const float4* __restrict__ pos_sorted, int n, Node* nodes, const int4* __restrict__ up,
+    float4* box_lo, float4* box_hi, const int* __restrict__ seg1,
+    const unsigned* __restrict__ cnt1, int nblocks1, int* seg2, unsigned* cnt2, int* error) {
+  __shared__ unsigned s_visit[kComSuper / 16];
+  __shared__ int s_front[kSeg];
+  __shared__ unsigned s_nfront;
+  for (int k = threadIdx.x; k < kComSuper / 16; k += blockDim.x) s_visit[k] = 0;
+  if (threadIdx.x == 0) s_nfront = 0;
+  __syncthreads();
+  const int lo = blockIdx.x * kComSuper;
+  const int hi = min(n - 1, lo + kComSuper - 1);
+  constexpr int kPer = kComSuper / kComLeaves;  // pass-1 workgroups per superblock
+  const int b0 = blockIdx.x * kPer;
+  auto counter = [&](int node) {
+    const int slot = node - lo;
+    const unsigned shift = 2u * (unsigned)(slot & 15);
+    const unsigned old = __hip_atomic_fetch_add(&s_visit[slot >> 4], 1u << shift,
+                                                __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+    return (old >> shift) & 3u;
+  };
+  // compact the segments' entries so that entry j goes to thread j
+  __shared__ unsigned s_off[kPer + 1];
+  if (threadIdx.x == 0) {
+    unsigned acc = 0;
+    for (int b = 0; b < kPer; ++b) {
+      s_off[b] = acc;
+      acc += (b0 + b < nblocks1) ? cnt1[b0 + b] : 0u;
+    }
+    s_off[kPer] = acc;
+  }
+  __syncthreads();
+  for (unsigned j = threadIdx.x; j < s_off[kPer]; j += blockDim.x) {
+    int b = 0;
+    for (int step = kPer / 2; step > 0; step >>= 1)  // last b with s_off[b] <= j
+      if (s_off[b + step] <= j) b += step;
+    climb_global(pos_sorted, n, nodes, up, box_lo, box_hi, seg1[(b0 + b) * kSeg + (j - s_off[b])],
+                 lo, hi, s_front, &s_nfront, counter, error);
+  }
+  __syncthreads();
+  const unsigned cnt = min(s_nfront, (unsigned)kSeg);
+  if (threadIdx.x < cnt) seg2[blockIdx.x * kSeg + threadIdx.x] = s_front[threadIdx.x];
+  if (threadIdx.x == 0) cnt2[blockIdx.x] = cnt;
+}
+
+// Pass 3: one workgroup climbs from every superblock's frontier segment
+// through the few nodes that span superblocks, with arrival counters in global
+// memory (one workgroup, so workgroup scope holds; earlier passes are visible
+// across the kernel boundaries).
 __global__ __launch_bounds__(kComThreads) void com_top_kernel(
-    const float4* __restrict__ pos_sorted, int n, Node* nodes, const int* __restrict__ parent,
-    unsigned* visit, float4* box_lo, float4* box_hi, const int* __restrict__ frontier,
-    const unsigned* frontier_count, int* error) {
-  const int nm1 = n - 1;
-  const unsigned nodes_total = 2u * (unsigned)n - 1u;
-  const unsigned count = *frontier_count;
-  for (unsigned e = threadIdx.x; e < count; e += blockDim.x) {
-    const int start = frontier[e];
-    if ((unsigned)start >= nodes_total) {
-      *error = 1;
-      continue;
+    const float4* __restrict__ pos_sorted, int n, Node* nodes, const int4* __restrict__ up,
+    unsigned* visit, float4* box_lo, float4* box_hi, const int* __restrict__ seg2,
+    const unsigned* __restrict__ cnt2, int nblocks2, int* error) {
+  __shared__ int s_dummy[1];
+  __shared__ unsigned s_dummy_cnt;
+  if (threadIdx.x == 0) s_dummy_cnt = 0;
+  __syncthreads();
+  auto counter = [&](int node) { return atomic_add_wg(&visit[node], 1u); };
+  // compact the entries of all superblock segments so entry j goes to thread j
+  __shared__ unsigned s_off[kTopSegs + 1];
+  if (threadIdx.x == 0) {
+    unsigned acc = 0;
+    for (int b = 0; b < nblocks2; ++b) {
+      s_off[b] = acc;
+      acc += cnt2[b];
     }
-    int node = parent[start];
-    while (node >= 0) {
-      if (node >= nm1) {
-        *error = 1;
-        break;
-      }
-      unsigned prev = atomic_add_wg(&visit[node], 1u);
-      if (prev != 1u) {
-        if (prev > 1u) *error = 1;
-        break;
-      }
-      if (!combine_node(pos_sorted, nm1, nodes_total, nodes, box_lo, box_hi, node)) {
-        *error = 1;
-        break;
-      }
-      node = parent[node];
-    }
+    s_off[nblocks2] = acc;
+  }
+  __syncthreads();
+  const unsigned total_entries = s_off[nblocks2];
+  for (unsigned j = threadIdx.x; j < total_entries; j += blockDim.x) {
+    int b = 0;
+    for (int step = kTopSegs / 2; step > 0; step >>= 1)  // last b with s_off[b] <= j
+      if (b + step <= nblocks2 && s_off[b + step] <= j) b += step;
+    // the whole tree is in range here, so no climb stops at a frontier
+    climb_global(pos_sorted, n, nodes, up, box_lo, box_hi, seg2[b * kSeg + (j - s_off[b])], 0,
+                 n - 1, s_dummy, &s_dummy_cnt, counter, error);
   }
 }
 
@@ -467,32 +611,44 @@ void launch_sort(hipStream_t s, int key_bits, void* keys_a, void* keys_b, unsign
                          max_tiles, first_pass_identity, gather_src, gather_dst, 8);
 }
 
-void launch_karras(hipStream_t s, int key_bits, const void* keys, int n, Node* nodes, int* parent,
-                   unsigned* visit, unsigned* frontier_count) {
+void launch_karras(hipStream_t s, int key_bits, const void* keys, int n, Node* nodes, int4* up,
+                   unsigned* visit) {
   if (n < 2) return;
   int g = (n - 1 + kBlock - 1) / kBlock;
   if (key_bits == 30)
-    karras_kernel<uint32_t><<<g, kBlock, 0, s>>>((const uint32_t*)keys, n, nodes, parent, visit,
-                                                 frontier_count);
+    karras_kernel<uint32_t><<<g, kBlock, 0, s>>>((const uint32_t*)keys, n, nodes, up, visit);
   else
-    karras_kernel<uint64_t><<<g, kBlock, 0, s>>>((const uint64_t*)keys, n, nodes, parent, visit,
-                                                 frontier_count);
+    karras_kernel<uint64_t><<<g, kBlock, 0, s>>>((const uint64_t*)keys, n, nodes, up, visit);
   NB_LAUNCH_CHECK();
 }
 
-void launch_com(hipStream_t s, const float4* pos_sorted, int n, Node* nodes, const int* parent,
-                unsigned* visit, float4* box_lo, float4* box_hi, int* frontier,
-                unsigned* frontier_count, int* error) {
+int com_scratch_ints(int n) {
+  const int g1 = (n + kComLeaves - 1) / kComLeaves;
+  const int g2 = (n + kComSuper - 1) / kComSuper;
+  return (g1 + g2) * (kSeg + 1);
+}
+
+void launch_com(hipStream_t s, const float4* pos_sorted, int n, Node* nodes, const int4* up,
+                unsigned* visit, float4* box_lo, float4* box_hi, int* scratch, int* error) {
   if (n < 2) return;
-  int g = (n + kComLeaves - 1) / kComLeaves;
-  com_local_kernel<<<g, kComThreads, 0, s>>>(pos_sorted, n, nodes, parent, box_lo, box_hi,
-                                             frontier, frontier_count, error);
+  const int g1 = (n + kComLeaves - 1) / kComLeaves;
+  const int g2 = (n + kComSuper - 1) / kComSuper;
+  int* seg1 = scratch;
+  int* seg2 = seg1 + g1 * kSeg;
+  unsigned* cnt1 = reinterpret_cast<unsigned*>(seg2 + g2 * kSeg);
+  unsigned* cnt2 = cnt1 + g1;
+  com_local_kernel<<<g1, kComThreads, 0, s>>>(pos_sorted, n, nodes, up, box_lo, box_hi, seg1,
+                                              cnt1, error);
   NB_LAUNCH_CHECK();
-  if (g > 1) {
-    com_top_kernel<<<1, kComThreads, 0, s>>>(pos_sorted, n, nodes, parent, visit, box_lo,
-                                             box_hi, frontier, frontier_count, error);
-    NB_LAUNCH_CHECK();
-  }
+  if (g1 == 1) return;
+  com_mid_kernel<<<g2, kComThreads, 0, s>>>(pos_sorted, n, nodes, up, box_lo, box_hi, seg1, cnt1,
+                                            g1, seg2, cnt2, error);
+  NB_LAUNCH_CHECK();
+  if (g2 == 1) return;
+  if (g2 > kTopSegs) throw Error(kErrInvalid, "particle count too large for the COM top pass");
+  com_top_kernel<<<1, kComThreads, 0, s>>>(pos_sorted, n, nodes, up, visit, box_lo, box_hi, seg2,
+                                           cnt2, g2, error);
+  NB_LAUNCH_CHECK();
 }
 
 }  // namespace nb

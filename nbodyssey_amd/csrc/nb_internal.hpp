@@ -6,7 +6,7 @@
 //   vals_a/vals_b        u32 x n, original index riding with each key
 //   pos_sorted           float4 x n, positions gathered into Morton order
 //   nodes                Node (32 B) x (n-1), one record per internal node
-//   parent               i32 x (2n-1)
+//   up                   int4 x (2n-1): parent, sibling, parent's range end (root: -1)
 //   visit                u32 x (n-1), bottom-up arrival counters
 //   box_lo/box_hi        float4 x (n-1), internal-node AABBs (used by the COM pass only)
 #pragma once
@@ -80,8 +80,7 @@ struct SortState {
   unsigned bbox_ticket;
   int overflow;             // traversal stack overflow flag
   int error;                // consistency guard tripped (bad sort/tree data)
-  unsigned frontier_count;  // COM: children of workgroup-spanning nodes
-  int pad[2];
+  int pad[3];
 };
 
 // ---------------------------------------------------------------- launchers
@@ -120,11 +119,12 @@ void launch_sort(hipStream_t s, int key_bits, void* keys_a, void* keys_b, unsign
                  float4* gather_dst);
 // (the sort sets SortState::error if a tile id or destination is out of range)
 int sort_passes(int key_bits);
-void launch_karras(hipStream_t s, int key_bits, const void* keys, int n, Node* nodes, int* parent,
-                   unsigned* visit, unsigned* frontier_count);
-void launch_com(hipStream_t s, const float4* pos_sorted, int n, Node* nodes, const int* parent,
-                unsigned* visit, float4* box_lo, float4* box_hi, int* frontier,
-                unsigned* frontier_count, int* error);
+void launch_karras(hipStream_t s, int key_bits, const void* keys, int n, Node* nodes, int4* up,
+                   unsigned* visit);
+// COM scratch: per-workgroup frontier segments and counts (ints)
+int com_scratch_ints(int n);
+void launch_com(hipStream_t s, const float4* pos_sorted, int n, Node* nodes, const int4* up,
+                unsigned* visit, float4* box_lo, float4* box_hi, int* scratch, int* error);
 
 // k_traverse.hip
 struct WalkArgs {

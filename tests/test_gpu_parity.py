@@ -34,6 +34,8 @@ def rel_err(a, b):
 CASES = {
     "blobs3024": lambda: blobs_with_dupes(),
     "disk65536": lambda: ic.two_disk_encounter(65536, seed=1, eps=0.02)[0],
+    # > 65,536 leaves: all three COM passes (block, superblock, top) run
+    "disk200k": lambda: ic.two_disk_encounter(200003, seed=2, eps=0.02)[0],
     "disk5000": lambda: ic.two_disk_encounter(5000, seed=7)[0],
     "n1": lambda: np.array([[0.5, -1.0, 2.0, 1.0]], np.float32),
     "n2": lambda: np.array([[0, 0, 0, 1], [1, 0, 0, 1]], np.float32),
@@ -77,7 +79,7 @@ def test_karras_bit_exact(case, bits):
 
 
 @pytest.mark.parametrize("bits", [63, 30])
-@pytest.mark.parametrize("case", ["blobs3024", "disk65536", "n2", "n3", "same"])
+@pytest.mark.parametrize("case", ["blobs3024", "disk65536", "disk200k", "n2", "n3", "same"])
 def test_tree_pipeline_bit_exact(case, bits):
     """bbox -> Morton -> sort -> gather -> Karras -> COM inside one force call."""
     pos = CASES[case]()
