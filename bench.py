@@ -125,6 +125,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=16384)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--accuracy-targets", type=int, default=256)
+    ap.add_argument("--graph", action="store_true",
+                    help="replay each step as a captured HIP graph (no per-phase timing)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -144,6 +146,8 @@ def main():
                             key_bits=args.key_bits)
     if args.force == "naive":
         return naive_main(args, sim, eps)
+    if args.graph:
+        sim.set_graph(True)
     sim.compute_forces()              # a(x0)
     sim.step(args.warmup)
     sim.synchronize()

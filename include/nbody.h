@@ -117,6 +117,12 @@ int nb_set_phase_timing(nb_engine* h, int on);
 /* bh_report_phase_timing data (src/forces_bh.cu:529-539): accumulated ms of
  * the six phases bbox, morton, sort, build, com, traverse and the call count. */
 int nb_bh_phase_ms(nb_engine* h, double out[6], long* calls);
+/* Graph mode off/on (default off).  On: nb_step replays one leapfrog step
+ * captured as a HIP graph (captured on first use, re-captured after any
+ * parameter change); results are bitwise identical to graph mode off.  Phase
+ * timing is not recorded in graph mode (the force-call total is: one call per
+ * step, kick and drift included).  No counterpart in the reference. */
+int nb_set_graph(nb_engine* h, int on);
 /* bh_release (src/forces_bh.cu:541-548): frees the tree scratch. */
 int nb_bh_release(nb_engine* h);
 

@@ -76,6 +76,7 @@ SIGNATURES = {
     "nb_reset_force_timing": (None, [_vp]),
     "nb_set_phase_timing": (_i, [_vp, _i]),
     "nb_bh_phase_ms": (_i, [_vp, _vp, _lp]),
+    "nb_set_graph": (_i, [_vp, _i]),
     "nb_bh_release": (_i, [_vp]),
     "nb_time_steps": (_i, [_vp, _i, _dp]),
     "nb_get_tree": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -42,7 +42,12 @@ struct DevBuf {
     if (c == 0) c = 1;
     NB_HIP(hipMalloc(reinterpret_cast<void**>(&p), c * sizeof(T)));
     count = c;
-    if (zero) NB_HIP(hipMemset(p, 0, c * sizeof(T)));
+    if (zero) {
+      // the engine's own stream is non-blocking and does not wait for the
+      // null stream, so finish the clear before anything else can touch p
+      NB_HIP(hipMemset(p, 0, c * sizeof(T)));
+      NB_HIP(hipDeviceSynchronize());
+    }
   }
   void reset() {
     if (p) (void)hipFree(p);
@@ -98,6 +103,7 @@ class Engine {
   // reference's default-stream semantics).
   Engine() : n(0) { check_device(); }
   ~Engine() {
+    drop_graph();
     for (auto& c : pending_) release_events(c);
     for (auto e : free_events_) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
@@ -105,6 +111,7 @@ class Engine {
 
   void bind(float4* p, float4* v, float4* a, int count) {
     if (count != n) acc_valid = false;
+    if (p != pos || v != vel || a != acc || count != n) drop_graph();
     pos = p;
     vel = v;
     acc = a;
@@ -114,6 +121,7 @@ class Engine {
   void set_params(const Params& np) {
     validate(np);
     if (np.key_bits != p.key_bits) release_bh();
+    if (std::memcmp(&np, &p, sizeof(Params)) != 0) drop_graph();
     p = np;
   }
 
@@ -128,21 +136,21 @@ class Engine {
     CallEvents ce{};
     const bool timed = timing_on;
     if (timed) begin_call(ce, bh && phase_timing);
+    prepare();
     if (bh && n >= 2) {
-      ensure_bh();
       const int passes = sort_passes(p.key_bits);
       if (step)
-        launch_kick_drift(stream, pos, vel, acc, n, half, p.dt, true, bh_partial.p, state.p,
-                          quant.p, passes);
+        launch_kick_drift(stream, pos, vel, acc, n, half, p.dt, true, bh_partial.p);
       else
-        launch_bbox(stream, pos, n, bh_partial.p, state.p, quant.p, passes);
+        launch_bbox(stream, pos, n, bh_partial.p);
       mark(ce, kPhBbox + 1);
-      launch_morton_hist(stream, pos, n, quant.p, p.key_bits, keys_a.p, state.p);
+      launch_morton_hist(stream, pos, n, bh_partial.p, bbox_parts(n), quant.p, p.key_bits,
+                         keys_a.p, state.p);
       mark(ce, kPhMorton + 1);
       launch_sort(stream, p.key_bits, keys_a.p, keys_b.p, vals_a.p, vals_b.p, n, state.p,
                   lookback.p, max_tiles, true, pos, pos_sorted.p);
       mark(ce, kPhSort + 1);
-      launch_karras(stream, p.key_bits, keys_a.p, n, nodes.p, up.p, visit.p);
+      launch_karras(stream, p.key_bits, keys_a.p, n, nodes.p, up.p, visit.p, state.p, passes);
       mark(ce, kPhBuild + 1);
       launch_com(stream, pos_sorted.p, n, nodes.p, up.p, visit.p, box_lo.p, box_hi.p,
                  frontier.p, &state.p->error);
@@ -166,15 +174,12 @@ class Engine {
       tree_valid = true;
     } else if (bh) {
       // n < 2: zero force (forces_bh.cu:466-469)
-      if (step) launch_kick_drift(stream, pos, vel, acc, n, half, p.dt, false, nullptr, nullptr,
-                                  nullptr, 0);
+      if (step) launch_kick_drift(stream, pos, vel, acc, n, half, p.dt, false, nullptr);
       launch_fill_u32(stream, reinterpret_cast<unsigned*>(acc), 0u, (size_t)n * 4);
       for (int k = 1; k <= kPhases; ++k) mark(ce, k);
     } else {
-      if (step) launch_kick_drift(stream, pos, vel, acc, n, half, p.dt, false, nullptr, nullptr,
-                                  nullptr, 0);
+      if (step) launch_kick_drift(stream, pos, vel, acc, n, half, p.dt, false, nullptr);
       int split = allpairs_split(n);
-      if (split > 1 && ap_partial.count < (size_t)split * n) ap_partial.alloc((size_t)split * n);
       launch_allpairs(stream, pos, n, eps2, p.G, acc, ap_partial.p, split, p.precision == 1,
                       step ? vel : nullptr, half);
       mark(ce, 1);
@@ -183,10 +188,61 @@ class Engine {
     acc_valid = true;
   }
 
+  // Allocations a force pass needs, done before (never during) graph capture.
+  void prepare() {
+    if (p.force == 1 && n >= 2) ensure_bh();
+    if (p.force == 0) {
+      const int split = allpairs_split(n);
+      if (split > 1 && ap_partial.count < (size_t)split * n) ap_partial.alloc((size_t)split * n);
+    }
+  }
+
   void step(int k) {
     if (k < 0) throw Error(kErrInvalid, "nsteps must be >= 0");
     if (k > 0 && !acc_valid) force_pass(false);
-    for (int s = 0; s < k; ++s) force_pass(true);
+    if (!graph_mode) {
+      for (int s = 0; s < k; ++s) force_pass(true);
+      return;
+    }
+    if (k > 0 && !step_graph) capture_step();
+    for (int s = 0; s < k; ++s) {
+      CallEvents ce{};
+      if (timing_on) begin_call(ce, false);
+      NB_HIP(hipGraphLaunch(step_graph, stream));
+      if (timing_on) end_call(ce);
+    }
+  }
+
+  // One leapfrog step captured as a HIP graph: the same kernels with the
+  // same arguments as force_pass(true); no memset nodes (the device-side
+  // bookkeeping is reset by the kernels themselves), no events inside.
+  void capture_step() {
+    prepare();
+    const bool saved = timing_on;
+    timing_on = false;
+    hipGraph_t g = nullptr;
+    NB_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    try {
+      force_pass(true);
+    } catch (...) {
+      (void)hipStreamEndCapture(stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      timing_on = saved;
+      throw;
+    }
+    NB_HIP(hipStreamEndCapture(stream, &g));
+    timing_on = saved;
+    hipError_t e = hipGraphInstantiate(&step_graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    NB_HIP(e);
+  }
+
+  void drop_graph() {
+    if (step_graph) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipGraphExecDestroy(step_graph);
+      step_graph = nullptr;
+    }
   }
 
   void check_overflow() {
@@ -240,9 +296,13 @@ class Engine {
     state.alloc(1, true);
     max_tiles = (n + kSortTile - 1) / kSortTile;
     lookback.alloc((size_t)kMaxPasses * max_tiles * 256, true);
+    // the sort state is reset by each step's Karras kernel for the next step;
+    // start from a reset state on this stream
+    launch_sort_init(stream, state.p, kMaxPasses);
     bh_cap = n;
   }
   void release_bh() {
+    drop_graph();
     keys_a.reset(); keys_b.reset(); vals_a.reset(); vals_b.reset(); pos_sorted.reset();
     nodes.reset(); up.reset(); visit.reset(); box_lo.reset(); box_hi.reset(); frontier.reset();
     bh_partial.reset(); quant.reset(); state.reset(); lookback.reset();
@@ -318,6 +378,8 @@ class Engine {
   bool tree_valid = false;
   bool timing_on = true;
   bool phase_timing = true;
+  bool graph_mode = false;
+  hipGraphExec_t step_graph = nullptr;
   double force_ms = 0.0;
   long force_calls = 0;
   double phase_ms[kPhases] = {};
@@ -571,6 +633,14 @@ int nb_bh_phase_ms(nb_engine* h, double out[6], long* calls) {
   });
 }
 
+int nb_set_graph(nb_engine* h, int on) {
+  return guarded([&] {
+    need(h, "engine");
+    h->e->drop_graph();
+    h->e->graph_mode = on != 0;
+  });
+}
+
 int nb_bh_release(nb_engine* h) {
   return guarded([&] {
     need(h, "engine");
@@ -697,8 +767,10 @@ int nb_morton(const float* pos4, int n, int key_bits, uint64_t* keys) {
     qp.alloc(1, true);
     ss.alloc(1, true);
     const int passes = nb::sort_passes(key_bits);
-    nb::launch_bbox(nullptr, dpos.p, n, part.p, ss.p, qp.p, passes);
-    nb::launch_morton_hist(nullptr, dpos.p, n, qp.p, key_bits, dkeys.p, ss.p);
+    (void)passes;
+    nb::launch_bbox(nullptr, dpos.p, n, part.p);
+    nb::launch_morton_hist(nullptr, dpos.p, n, part.p, nb::bbox_parts(n), qp.p, key_bits,
+                           dkeys.p, ss.p);
     NB_HIP(hipDeviceSynchronize());
     if (key_bits == 30) {
       std::vector<uint32_t> k(n);
@@ -779,7 +851,7 @@ int nb_karras_build(const uint64_t* keys_sorted, int n, int key_bits, int32_t* c
     nodes.alloc(n - 1);
     par.alloc(2 * (size_t)n - 1);
     visit.alloc(n - 1);
-    nb::launch_karras(nullptr, key_bits, dk.p, n, nodes.p, par.p, visit.p);
+    nb::launch_karras(nullptr, key_bits, dk.p, n, nodes.p, par.p, visit.p, nullptr, 0);
     NB_HIP(hipDeviceSynchronize());
     if (parent) {
       std::vector<int4> u(2 * (size_t)n - 1);

@@ -1,9 +1,10 @@
 // Leapfrog kernels and the bounding-box reduction.
 //
 // kick_drift fuses the first kick and the drift of leapfrog_step
-// (integrator.cu:5-33) and, for Barnes-Hut, the bounding-box reduction of the
-// new positions (forces_bh.cu:125-188), so one pass over pos/vel/acc does all
-// three.  The closing kick (integrator.cu:35) is fused into the force
+// (integrator.cu:5-33) and, for Barnes-Hut, the per-workgroup bounding boxes
+// of the new positions (forces_bh.cu:125-156), so one pass over pos/vel/acc
+// does all three; the Morton kernel finishes the reduction
+// (forces_bh.cu:158-188).  The closing kick (integrator.cu:35) is fused into the force
 // kernels' epilogues.  The FP32 operations are the reference's:
 //   v += (dt/2) a;  x += dt v   (as fmaf), mass in pos.w untouched.
 #include "nb_device.hpp"
@@ -14,14 +15,12 @@
 namespace nb {
 namespace {
 
-// Block reduction of the per-thread boxes, then the last block to finish
-// reduces all block partials, writes the FP64 quantisation constants and
-// resets the sort bookkeeping for the sort that follows.
-__device__ void bbox_reduce_and_finalize(float lo[3], float hi[3], float4* partial,
-                                         SortState* ss, QuantParams* qp, int passes) {
+// Block reduction of the per-thread boxes; thread 0 writes the block's
+// (lo, hi) pair.  The Morton kernel reduces the pairs (kernel boundary =
+// visibility), so no cross-workgroup synchronisation is needed here.
+__device__ void bbox_block_partial(float lo[3], float hi[3], float4* partial) {
   constexpr int W = kBlock / 64;
   __shared__ float s_lo[W][3], s_hi[W][3];
-  __shared__ int s_last;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -44,64 +43,14 @@ __device__ void bbox_reduce_and_finalize(float lo[3], float hi[3], float4* parti
         bh[a] = fmaxf(bh[a], s_hi[k][a]);
       }
     }
-    unsigned* pp = reinterpret_cast<unsigned*>(partial + 2 * blockIdx.x);
-    for (int a = 0; a < 3; ++a) {
-      store_agent_u32(pp + a, __float_as_uint(bl[a]));
-      store_agent_u32(pp + 4 + a, __float_as_uint(bh[a]));
-    }
-    unsigned prev = atomic_add_agent(&ss->bbox_ticket, 1u);
-    s_last = (prev == gridDim.x - 1);
+    partial[2 * blockIdx.x] = make_float4(bl[0], bl[1], bl[2], 0.f);
+    partial[2 * blockIdx.x + 1] = make_float4(bh[0], bh[1], bh[2], 0.f);
   }
-  __syncthreads();
-  if (!s_last) return;
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-
-  float l[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, h[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
-    unsigned* pp = reinterpret_cast<unsigned*>(partial + 2 * b);
-    for (int a = 0; a < 3; ++a) {
-      l[a] = fminf(l[a], __uint_as_float(load_agent_u32(pp + a)));
-      h[a] = fmaxf(h[a], __uint_as_float(load_agent_u32(pp + 4 + a)));
-    }
-  }
-  __syncthreads();  // s_lo/s_hi reuse
-  for (int a = 0; a < 3; ++a) {
-    l[a] = wave_min(l[a]);
-    h[a] = wave_max(h[a]);
-  }
-  if (lane == 0)
-    for (int a = 0; a < 3; ++a) {
-      s_lo[w][a] = l[a];
-      s_hi[w][a] = h[a];
-    }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double ext = 0.0;
-    for (int a = 0; a < 3; ++a) {
-      float bl = s_lo[0][a], bh = s_hi[0][a];
-      for (int k = 1; k < W; ++k) {
-        bl = fminf(bl, s_lo[k][a]);
-        bh = fmaxf(bh, s_hi[k][a]);
-      }
-      qp->lo[a] = (double)bl;
-      qp->box_lo[a] = bl;
-      qp->box_hi[a] = bh;
-      double e = (double)bh - (double)bl;
-      if (e > ext) ext = e;
-    }
-    qp->box_lo[3] = 0.f;
-    qp->box_hi[3] = 0.f;
-    qp->inv = ext > 0.0 ? 1.0 / ext : 0.0;
-    ss->bbox_ticket = 0;
-    ss->epoch = ss->epoch + 1;
-    for (int p = 0; p < kMaxPasses; ++p) ss->ticket[p] = 0;
-  }
-  for (int p = 0; p < passes; ++p) ss->hist[p][threadIdx.x] = 0;
 }
 
 __global__ __launch_bounds__(kBlock) void kick_drift_bbox_kernel(
     float4* __restrict__ pos, float4* __restrict__ vel, const float4* __restrict__ acc, int n,
-    float half_dt, float dt, float4* partial, SortState* ss, QuantParams* qp, int passes) {
+    float half_dt, float dt, float4* partial) {
   float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     float4 v = vel[i], a = acc[i], p = pos[i];
@@ -116,7 +65,7 @@ __global__ __launch_bounds__(kBlock) void kick_drift_bbox_kernel(
     lo[0] = fminf(lo[0], p.x); lo[1] = fminf(lo[1], p.y); lo[2] = fminf(lo[2], p.z);
     hi[0] = fmaxf(hi[0], p.x); hi[1] = fmaxf(hi[1], p.y); hi[2] = fmaxf(hi[2], p.z);
   }
-  bbox_reduce_and_finalize(lo, hi, partial, ss, qp, passes);
+  bbox_block_partial(lo, hi, partial);
 }
 
 __global__ __launch_bounds__(kBlock) void kick_drift_kernel(float4* __restrict__ pos,
@@ -137,15 +86,14 @@ __global__ __launch_bounds__(kBlock) void kick_drift_kernel(float4* __restrict__
 }
 
 __global__ __launch_bounds__(kBlock) void bbox_kernel(const float4* __restrict__ pos, int n,
-                                                      float4* partial, SortState* ss,
-                                                      QuantParams* qp, int passes) {
+                                                      float4* partial) {
   float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     float4 p = pos[i];
     lo[0] = fminf(lo[0], p.x); lo[1] = fminf(lo[1], p.y); lo[2] = fminf(lo[2], p.z);
     hi[0] = fmaxf(hi[0], p.x); hi[1] = fmaxf(hi[1], p.y); hi[2] = fmaxf(hi[2], p.z);
   }
-  bbox_reduce_and_finalize(lo, hi, partial, ss, qp, passes);
+  bbox_block_partial(lo, hi, partial);
 }
 
 __global__ __launch_bounds__(kBlock) void kick_kernel(float4* __restrict__ vel,
@@ -165,20 +113,19 @@ __global__ void fill_u32_kernel(unsigned* p, unsigned v, size_t count) {
   for (; i < count; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
-int bbox_grid(int n) {
+}  // namespace
+
+int bbox_parts(int n) {
   int g = (n + kBlock - 1) / kBlock;
   return g < kBboxBlocks ? (g > 0 ? g : 1) : kBboxBlocks;
 }
 
-}  // namespace
-
 void launch_kick_drift(hipStream_t s, float4* pos, float4* vel, const float4* acc, int n,
-                       float half_dt, float dt, bool with_bbox, float4* partial, SortState* ss,
-                       QuantParams* qp, int passes) {
+                       float half_dt, float dt, bool with_bbox, float4* partial) {
   if (n <= 0) return;
   if (with_bbox) {
-    kick_drift_bbox_kernel<<<bbox_grid(n), kBlock, 0, s>>>(pos, vel, acc, n, half_dt, dt,
-                                                           partial, ss, qp, passes);
+    kick_drift_bbox_kernel<<<bbox_parts(n), kBlock, 0, s>>>(pos, vel, acc, n, half_dt, dt,
+                                                            partial);
   } else {
     kick_drift_kernel<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(pos, vel, acc, n, half_dt,
                                                                    dt);
@@ -186,10 +133,9 @@ void launch_kick_drift(hipStream_t s, float4* pos, float4* vel, const float4* ac
   NB_LAUNCH_CHECK();
 }
 
-void launch_bbox(hipStream_t s, const float4* pos, int n, float4* partial, SortState* ss,
-                 QuantParams* qp, int passes) {
+void launch_bbox(hipStream_t s, const float4* pos, int n, float4* partial) {
   if (n <= 0) return;
-  bbox_kernel<<<bbox_grid(n), kBlock, 0, s>>>(pos, n, partial, ss, qp, passes);
+  bbox_kernel<<<bbox_parts(n), kBlock, 0, s>>>(pos, n, partial);
   NB_LAUNCH_CHECK();
 }
 

@@ -58,13 +58,60 @@ __device__ __forceinline__ K make_key(float4 p, double lx, double ly, double lz,
 
 template <typename K>
 __global__ __launch_bounds__(kBlock) void morton_hist_kernel(const float4* __restrict__ pos, int n,
-                                                             const QuantParams* __restrict__ qp,
+                                                             const float4* __restrict__ partial,
+                                                             int nparts, QuantParams* qp,
                                                              K* __restrict__ keys, SortState* ss,
                                                              int passes) {
   __shared__ unsigned h[kMaxPasses][256];
+  __shared__ float s_box[kBlock / 64][6];
+  __shared__ double s_q[4];
   for (int p = 0; p < passes; ++p) h[p][threadIdx.x] = 0;
+  // global bounding box from the per-workgroup partials (forces_bh.cu:158-188)
+  float l[3] = {3.402823466e38f, 3.402823466e38f, 3.402823466e38f};
+  float u[3] = {-3.402823466e38f, -3.402823466e38f, -3.402823466e38f};
+  for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
+    const float4 a = partial[2 * k], b = partial[2 * k + 1];
+    l[0] = fminf(l[0], a.x); l[1] = fminf(l[1], a.y); l[2] = fminf(l[2], a.z);
+    u[0] = fmaxf(u[0], b.x); u[1] = fmaxf(u[1], b.y); u[2] = fmaxf(u[2], b.z);
+  }
+  for (int a = 0; a < 3; ++a) {
+    l[a] = wave_min(l[a]);
+    u[a] = wave_max(u[a]);
+  }
+  if ((threadIdx.x & 63) == 0)
+    for (int a = 0; a < 3; ++a) {
+      s_box[threadIdx.x >> 6][a] = l[a];
+      s_box[threadIdx.x >> 6][3 + a] = u[a];
+    }
   __syncthreads();
-  const double lx = qp->lo[0], ly = qp->lo[1], lz = qp->lo[2], inv = qp->inv;
+  if (threadIdx.x == 0) {
+    // FP64 origin and cubic scale (lbvh_check.py:45-48): lo, inv = 1/max(hi-lo)
+    double ext = 0.0;
+    float bl[3], bh[3];
+    for (int a = 0; a < 3; ++a) {
+      bl[a] = s_box[0][a];
+      bh[a] = s_box[0][3 + a];
+      for (int w = 1; w < kBlock / 64; ++w) {
+        bl[a] = fminf(bl[a], s_box[w][a]);
+        bh[a] = fmaxf(bh[a], s_box[w][3 + a]);
+      }
+      s_q[a] = (double)bl[a];
+      const double e = (double)bh[a] - (double)bl[a];
+      if (e > ext) ext = e;
+    }
+    s_q[3] = ext > 0.0 ? 1.0 / ext : 0.0;
+    if (blockIdx.x == 0) {
+      for (int a = 0; a < 3; ++a) {
+        qp->lo[a] = s_q[a];
+        qp->box_lo[a] = bl[a];
+        qp->box_hi[a] = bh[a];
+      }
+      qp->inv = s_q[3];
+      qp->box_lo[3] = qp->box_hi[3] = 0.f;
+    }
+  }
+  __syncthreads();
+  const double lx = s_q[0], ly = s_q[1], lz = s_q[2], inv = s_q[3];
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     K k = make_key<K>(pos[i], lx, ly, lz, inv);
     keys[i] = k;
@@ -232,8 +279,15 @@ template <typename K>
 __global__ __launch_bounds__(kBlock) void karras_kernel(const K* __restrict__ keys, int n,
                                                         Node* __restrict__ nodes,
                                                         int4* __restrict__ up,
-                                                        unsigned* __restrict__ visit) {
+                                                        unsigned* __restrict__ visit,
+                                                        SortState* ss, int passes) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ss && blockIdx.x == 0) {
+    // the sort of this step is complete: reset its state for the next one
+    for (int p = 0; p < passes; ++p) ss->hist[p][threadIdx.x] = 0;
+    if (threadIdx.x < kMaxPasses) ss->ticket[threadIdx.x] = 0;
+    if (threadIdx.x == 0) ss->epoch = ss->epoch + 1;
+  }
   if (i >= n - 1) return;
   const K ki = keys[i];
   int d = (delta(keys, n, i, ki, i + 1) - delta(keys, n, i, ki, i - 1)) >= 0 ? 1 : -1;
@@ -572,14 +626,16 @@ void sort_typed(hipStream_t s, K* ka, K* kb, unsigned* va, unsigned* vb, int n, 
 
 int sort_passes(int key_bits) { return key_bits == 30 ? 4 : 8; }
 
-void launch_morton_hist(hipStream_t s, const float4* pos, int n, const QuantParams* qp,
-                        int key_bits, void* keys, SortState* ss) {
+void launch_morton_hist(hipStream_t s, const float4* pos, int n, const float4* partial,
+                        int nparts, QuantParams* qp, int key_bits, void* keys, SortState* ss) {
   if (n <= 0) return;
   int g = grid_for(n, 8, 2048);
   if (key_bits == 30)
-    morton_hist_kernel<uint32_t><<<g, kBlock, 0, s>>>(pos, n, qp, (uint32_t*)keys, ss, 4);
+    morton_hist_kernel<uint32_t><<<g, kBlock, 0, s>>>(pos, n, partial, nparts, qp,
+                                                      (uint32_t*)keys, ss, 4);
   else
-    morton_hist_kernel<uint64_t><<<g, kBlock, 0, s>>>(pos, n, qp, (uint64_t*)keys, ss, 8);
+    morton_hist_kernel<uint64_t><<<g, kBlock, 0, s>>>(pos, n, partial, nparts, qp,
+                                                      (uint64_t*)keys, ss, 8);
   NB_LAUNCH_CHECK();
 }
 
@@ -612,13 +668,15 @@ void launch_sort(hipStream_t s, int key_bits, void* keys_a, void* keys_b, unsign
 }
 
 void launch_karras(hipStream_t s, int key_bits, const void* keys, int n, Node* nodes, int4* up,
-                   unsigned* visit) {
+                   unsigned* visit, SortState* ss, int passes) {
   if (n < 2) return;
   int g = (n - 1 + kBlock - 1) / kBlock;
   if (key_bits == 30)
-    karras_kernel<uint32_t><<<g, kBlock, 0, s>>>((const uint32_t*)keys, n, nodes, up, visit);
+    karras_kernel<uint32_t><<<g, kBlock, 0, s>>>((const uint32_t*)keys, n, nodes, up, visit, ss,
+                                                 passes);
   else
-    karras_kernel<uint64_t><<<g, kBlock, 0, s>>>((const uint64_t*)keys, n, nodes, up, visit);
+    karras_kernel<uint64_t><<<g, kBlock, 0, s>>>((const uint64_t*)keys, n, nodes, up, visit, ss,
+                                                 passes);
   NB_LAUNCH_CHECK();
 }
 

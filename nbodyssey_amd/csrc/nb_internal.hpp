@@ -47,7 +47,7 @@ enum Status {
 
 constexpr int kBlock = 256;          // kForceBlock (forces.cuh:25)
 constexpr int kStackDepth = 128;     // kBhStack (forces_bh.cu:43)
-constexpr int kBboxBlocks = 512;     // grid of the bbox / kick-drift-bbox reductions
+constexpr int kBboxBlocks = 1024;    // max partial boxes written by the bbox / kick-drift-bbox pass
 constexpr int kSortItems = 16;       // keys per thread per onesweep tile
 constexpr int kSortTile = kBlock * kSortItems;
 constexpr int kMaxPasses = 8;
@@ -77,7 +77,7 @@ struct SortState {
   unsigned hist[kMaxPasses][256];
   unsigned ticket[kMaxPasses];
   unsigned epoch;
-  unsigned bbox_ticket;
+  unsigned pad0;
   int overflow;             // traversal stack overflow flag
   int error;                // consistency guard tripped (bad sort/tree data)
   int pad[3];
@@ -87,11 +87,12 @@ struct SortState {
 // Every launcher enqueues on `s` and checks the launch.
 
 // k_integrate.hip
+// The bbox passes write one (lo, hi) pair per workgroup into `partial`
+// (bbox_parts(n) pairs); the Morton kernel reduces them.
+int bbox_parts(int n);
 void launch_kick_drift(hipStream_t s, float4* pos, float4* vel, const float4* acc, int n,
-                       float half_dt, float dt, bool with_bbox, float4* partial,
-                       SortState* ss, QuantParams* qp, int passes);
-void launch_bbox(hipStream_t s, const float4* pos, int n, float4* partial, SortState* ss,
-                 QuantParams* qp, int passes);
+                       float half_dt, float dt, bool with_bbox, float4* partial);
+void launch_bbox(hipStream_t s, const float4* pos, int n, float4* partial);
 void launch_kick(hipStream_t s, float4* vel, const float4* acc, int n, float half_dt);
 void launch_fill_u32(hipStream_t s, unsigned* p, unsigned v, size_t count);
 
@@ -105,8 +106,12 @@ void launch_energy(hipStream_t s, const float4* pos, const float4* vel, int n, f
 int energy_partials(int n);
 
 // k_lbvh.hip
-void launch_morton_hist(hipStream_t s, const float4* pos, int n, const QuantParams* qp,
-                        int key_bits, void* keys, SortState* ss);
+// Reduces the bbox partials, forms the FP64 quantisation (block 0 stores it
+// in *qp), writes the keys and accumulates every pass's digit histogram.
+// The sort state must be reset (launch_sort_init, or the previous step's
+// Karras kernel via launch_karras(..., ss, passes)).
+void launch_morton_hist(hipStream_t s, const float4* pos, int n, const float4* partial,
+                        int nparts, QuantParams* qp, int key_bits, void* keys, SortState* ss);
 void launch_hist(hipStream_t s, const void* keys, int n, int key_bits, SortState* ss);
 void launch_sort_init(hipStream_t s, SortState* ss, int passes);
 // Sorts (keys_a, vals) in place through the b buffers; result lands in *_a for
@@ -119,8 +124,10 @@ void launch_sort(hipStream_t s, int key_bits, void* keys_a, void* keys_b, unsign
                  float4* gather_dst);
 // (the sort sets SortState::error if a tile id or destination is out of range)
 int sort_passes(int key_bits);
+// Also resets `ss` (histograms of `passes` passes, tickets, epoch) for the
+// next sort when ss is not null.
 void launch_karras(hipStream_t s, int key_bits, const void* keys, int n, Node* nodes, int4* up,
-                   unsigned* visit);
+                   unsigned* visit, SortState* ss, int passes);
 // COM scratch: per-workgroup frontier segments and counts (ints)
 int com_scratch_ints(int n);
 void launch_com(hipStream_t s, const float4* pos_sorted, int n, Node* nodes, const int4* up,

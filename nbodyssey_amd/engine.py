@@ -138,6 +138,10 @@ class Simulation:
         check(self._L.nb_bh_phase_ms(self._h, ptr(out), ctypes.byref(calls)))
         return out, calls.value
 
+    def set_graph(self, on):
+        """Replay each leapfrog step as a captured HIP graph."""
+        check(self._L.nb_set_graph(self._h, 1 if on else 0))
+
     def bh_release(self):
         check(self._L.nb_bh_release(self._h))
 
