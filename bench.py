@@ -109,6 +109,119 @@ def naive_main(args, sim, eps):
     sim.close()
 
 
+def ensemble_main(args):
+    """Batched ensemble of independent parabolic two-disk encounters against
+    the same systems run one at a time on the plain path.
+
+    Batched: one engine (nb_create_ensemble), every step captured as a HIP
+    graph.  Serial: one plain engine per system (Barnes-Hut, wave64 walk,
+    30-bit keys, graph mode: the fastest plain configuration), started from
+    the batched run's final state, timed per system and summed.  Both compute
+    bitwise the same forces for a given state (the forest holds each system's
+    standalone tree), so the accuracy is the same by construction.
+    """
+    from nbodyssey_amd import engine, ic
+    w = ic.WORKLOADS["ensemble-256-parabolic"]
+    eps, dt, theta = w["eps"], w["dt"], args.theta
+    t_ic = time.perf_counter()
+    systems, _ = ic.ensemble_encounters(args.systems, seed=w["seed"])
+    counts = np.array([len(p) for p, _ in systems])
+    n = int(counts.sum())
+    print(f"bench: ensemble {args.systems} systems, {n} particles, ic "
+          f"{time.perf_counter() - t_ic:.1f}s", file=sys.stderr, flush=True)
+
+    ens = engine.Ensemble(systems, dt=dt, eps=eps, theta=theta)
+    ens.set_graph(True)
+    ens.compute_forces()
+    ens.step(args.warmup)
+    ens.synchronize()
+    ms_batched = ens.time_steps(args.steps)
+    print(f"bench: batched {args.steps} steps {ms_batched:.2f} ms", file=sys.stderr, flush=True)
+    # phase times (walk kernel duration) from eager steps of the same engine
+    ens.set_graph(False)
+    ens.reset_force_timing()
+    ens.time_steps(args.phase_steps)
+    phase, calls = ens.bh_phase_ms()
+    walk_ms = phase[5] / max(calls, 1)
+    acc_b = ens.acc()
+    p_b, v_b = ens.download()
+    off = ens.offsets
+    ens.close()
+
+    # serial: each system alone from the batched run's final state
+    ms_serial = 0.0
+    mismatched = []
+    inter_sum = opens_sum = 0.0
+    err = []
+    rng = np.random.default_rng(0)
+    per_sys = max(1, args.accuracy_targets // args.systems)
+    for s in range(args.systems):
+        o, m = int(off[s]), int(counts[s])
+        with engine.Simulation(p_b[o:o + m], v_b[o:o + m], dt=dt, eps=eps, theta=theta,
+                               force="bh", traverse="warp", group_width=64,
+                               key_bits=30) as sim:
+            a_s = sim.compute_forces()
+            if not np.array_equal(a_s, acc_b[o:o + m]):
+                mismatched.append(s)
+            ti, to = sim.traverse_counts("thread")
+            inter_sum += ti.astype(np.float64).sum()
+            opens_sum += to.astype(np.float64).sum()
+            try:
+                from oracle import orc
+                idx = rng.choice(m, per_sys, replace=False).astype(np.int32)
+                ref = orc.allpairs_f64(p_b[o:o + m], eps, targets=idx)
+                err.append(np.linalg.norm(a_s[idx, :3] - ref, axis=1) /
+                           np.linalg.norm(ref, axis=1))
+            except Exception as exc:  # the oracle is a checker only
+                err = None
+                print(f"bench: accuracy check skipped: {exc}", file=sys.stderr)
+            sim.set_graph(True)
+            sim.step(args.serial_warmup)
+            sim.synchronize()
+            ms_serial += sim.time_steps(args.serial_steps) / args.serial_steps
+    ms_serial_total = ms_serial * args.steps   # same step count as the batched run
+    value = n * args.steps / (ms_batched * 1e-3)
+    serial_value = n * args.steps / (ms_serial_total * 1e-3)
+    print(f"bench: serial {ms_serial:.3f} ms per ensemble step (sum over systems)",
+          file=sys.stderr, flush=True)
+    flops = 20.0 * inter_sum + 10.0 * opens_sum
+    achieved = flops / (walk_ms * 1e-3) / 1e12
+    names = ["bbox", "morton", "sort", "build", "com", "traverse"]
+    accuracy = None
+    if err:
+        e = np.concatenate(err)
+        accuracy = {"median_rel_force_err_vs_fp64": float(np.median(e)),
+                    "p99_rel_force_err_vs_fp64": float(np.quantile(e, 0.99)),
+                    "targets": int(len(e))}
+    out = {
+        "metric": "ensemble particle-steps/s (batched vs serial)",
+        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_batched / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic parabolic two-disk encounters (seeded generator)",
+        "config": {"workload": f"ensemble of {args.systems} two-disk encounters, {n} particles "
+                               f"(n_s log-uniform 2048..65536), Barnes-Hut rebuilt every step, "
+                               f"(system, 30-bit) keys, theta={theta}, eps={eps}, dt={dt}, "
+                               f"wave64 persistent walk, graph-captured step",
+                   "systems": args.systems, "n": n, "theta": theta},
+        "serial": {"value": serial_value, "ms_per_ensemble_step": ms_serial,
+                   "how": "each system alone on the plain path (bh, wave64, 30-bit keys, "
+                          "graph mode), summed"},
+        "batched_over_serial": value / serial_value,
+        "batched_forces_bitwise_equal_serial": not mismatched,
+        "phase_ms": {k: phase[i] / max(calls, 1) for i, k in enumerate(names)},
+        "roofline": {"bound": "valu", "kernel": "walk_items",
+                     "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "flops_per_launch": flops, "walk_ms": walk_ms,
+                     "note": f"walk duration from {args.phase_steps} eager steps after the "
+                             f"timed graph steps"},
+        "accuracy": accuracy,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,12 +240,24 @@ def main():
     ap.add_argument("--accuracy-targets", type=int, default=256)
     ap.add_argument("--graph", action="store_true",
                     help="replay each step as a captured HIP graph (no per-phase timing)")
+    ap.add_argument("--workload", default="two-disk", choices=["two-disk", "ensemble"],
+                    help="ensemble: batched independent systems vs the same systems one at a "
+                         "time (defaults: 256 systems, 200 steps)")
+    ap.add_argument("--systems", type=int, default=256)
+    ap.add_argument("--serial-steps", type=int, default=20)
+    ap.add_argument("--serial-warmup", type=int, default=2)
+    ap.add_argument("--phase-steps", type=int, default=10)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the hot path (one N-body system per GPU) has "
               "no multi-GPU mode in this build; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "ensemble":
+        if "--steps" not in sys.argv:
+            args.steps = 200
+        return ensemble_main(args)
 
     from nbodyssey_amd import engine, ic
     eps, dt = 0.02, 0.005

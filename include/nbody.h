@@ -157,6 +157,39 @@ int nb_sort_pairs(const uint64_t* keys, int n, int key_bits, uint64_t* keys_sort
 int nb_karras_build(const uint64_t* keys_sorted, int n, int key_bits, int32_t* children,
                     int32_t* parent);
 
+/* ---------------------------------------------------------------- ensembles
+ * No counterpart in the reference, which runs one system per process
+ * (main.cu:72-240); additive, for batched runs of many independent systems.
+ * The systems are stored back to back: system s holds particles
+ * [offsets[s], offsets[s+1]) of the engine's arrays, so nb_upload,
+ * nb_download, nb_compute_forces, nb_step and nb_set_graph work on all of
+ * them at once.  Each step runs the single-system pipeline once over the
+ * whole ensemble with (system << 30 | 30-bit Morton key) sort keys, a forest
+ * build in which every system's subtree equals its standalone tree, and a
+ * persistent wave64 walk pulling (system, 64-target block) items from a
+ * device queue.  Parameters must be force = NB_FORCE_BH, traverse =
+ * NB_TRAVERSE_WARP, group_width 64, key_bits 30; dt, epsilon, G, theta and
+ * precision apply to every system. */
+
+/* nsys in [1, 1024] systems of counts[s] >= 2 particles (total <= 2^30). */
+int nb_create_ensemble(int nsys, const int* counts, nb_engine** out);
+/* Number of systems (0 for a single-system engine, -1 for NULL). */
+int nb_ensemble_systems(const nb_engine* h);
+/* nsys + 1 offsets into the particle arrays. */
+int nb_ensemble_offsets(const nb_engine* h, int32_t* offsets);
+/* Walk work items in queue order, 4 ints each: (system, first sorted target,
+ * target count <= 64, 0); *nitems receives their number.  items4 may be NULL.
+ * nb_traverse_counts on an ensemble (traverse 1, width 64) reports one count
+ * per item in this order. */
+int nb_ensemble_items(const nb_engine* h, int32_t* items4, int* nitems);
+/* Internal node id of each system's subtree root in the last forest built
+ * (nb_get_tree returns the forest: sorted keys, perm, children, ...). */
+int nb_ensemble_roots(nb_engine* h, int32_t* roots);
+/* total_energy (src/integrator.cu:43-91) of every system on its own:
+ * 3 doubles (total, kinetic, potential) per system.  nb_total_energy on an
+ * ensemble returns the sums of these (no energy between systems). */
+int nb_system_energy(nb_engine* h, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -84,6 +84,12 @@ SIGNATURES = {
     "nb_morton": (_i, [_vp, _i, _i, _vp]),
     "nb_sort_pairs": (_i, [_vp, _i, _i, _vp, _vp]),
     "nb_karras_build": (_i, [_vp, _i, _i, _vp, _vp]),
+    "nb_create_ensemble": (_i, [_i, _vp, ctypes.POINTER(_vp)]),
+    "nb_ensemble_systems": (_i, [_vp]),
+    "nb_ensemble_offsets": (_i, [_vp, _vp]),
+    "nb_ensemble_items": (_i, [_vp, _vp, ctypes.POINTER(_i)]),
+    "nb_ensemble_roots": (_i, [_vp, _vp]),
+    "nb_system_energy": (_i, [_vp, _vp]),
 }
 
 _lib = None

@@ -19,6 +19,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <string>
 #include <vector>
 
 namespace nb {
@@ -120,6 +121,11 @@ class Engine {
 
   void set_params(const Params& np) {
     validate(np);
+    if (nsys > 0 && (np.force != 1 || np.traverse != 1 || np.group_width != 64 ||
+                     np.key_bits != 30))
+      throw Error(kErrInvalid,
+                  "an ensemble runs Barnes-Hut with the wave64 walk on (system, 30-bit) keys: "
+                  "force=bh, traverse=warp, group_width=64, key_bits=30");
     if (np.key_bits != p.key_bits) release_bh();
     if (std::memcmp(&np, &p, sizeof(Params)) != 0) drop_graph();
     p = np;
@@ -137,7 +143,9 @@ class Engine {
     const bool timed = timing_on;
     if (timed) begin_call(ce, bh && phase_timing);
     prepare();
-    if (bh && n >= 2) {
+    if (nsys > 0) {
+      ens_force_pass(step, ce);
+    } else if (bh && n >= 2) {
       const int passes = sort_passes(p.key_bits);
       if (step)
         launch_kick_drift(stream, pos, vel, acc, n, half, p.dt, true, bh_partial.p);
@@ -188,9 +196,122 @@ class Engine {
     acc_valid = true;
   }
 
+  // One force evaluation over every system of the ensemble: the same pipeline
+  // as a single system with (system, Morton) keys, a forest build and the
+  // persistent walk over (system, target block) items.
+  void ens_force_pass(bool step, CallEvents& ce) {
+    const float half = 0.5f * p.dt;
+    const int passes = ens_passes();
+    launch_ens_kick_drift_bbox(stream, pos, vel, acc, n, half, p.dt, step, sys_off.p, nsys,
+                               sys_box.p);
+    mark(ce, kPhBbox + 1);
+    launch_ens_morton_hist(stream, pos, n, sys_off.p, nsys, sys_box.p, keys_a.p, state.p);
+    mark(ce, kPhMorton + 1);
+    launch_sort_u64(stream, keys_a.p, keys_b.p, vals_a.p, vals_b.p, n, passes, state.p,
+                    lookback.p, max_tiles, true, pos, pos_sorted.p);
+    mark(ce, kPhSort + 1);
+    launch_ens_karras(stream, keys_final(), n, sys_off.p, nsys, nodes.p, up.p, visit.p, roots.p,
+                      state.p, sys_box.p);
+    mark(ce, kPhBuild + 1);
+    launch_com(stream, pos_sorted.p, n, nodes.p, up.p, visit.p, box_lo.p, box_hi.p, frontier.p,
+               &state.p->error);
+    mark(ce, kPhCom + 1);
+    WalkArgs w = walk_args();
+    w.vel = step ? vel : nullptr;
+    w.half_dt = half;
+    launch_walk_items(stream, w, items.p, nitems, roots.p, &state.p->queue, 0);
+    mark(ce, kPhWalk + 1);
+    tree_valid = true;
+  }
+
+  WalkArgs walk_args() {
+    WalkArgs w{};
+    w.pos_sorted = pos_sorted.p;
+    w.perm = perm_final();
+    w.n = n;
+    w.nodes = nodes.p;
+    w.theta2 = p.theta * p.theta;
+    w.eps2 = p.eps * p.eps;
+    w.G = p.G;
+    w.acc = acc;
+    w.overflow = &state.p->overflow;
+    w.error = &state.p->error;
+    w.compensated = p.precision == 1;
+    return w;
+  }
+
+  // Sorted keys / original indices after the last sort pass (an ensemble
+  // sorts an odd number of passes and ends in the b buffers).
+  uint64_t* keys_final() { return (nsys > 0 && (ens_passes() & 1)) ? keys_b.p : keys_a.p; }
+  unsigned* perm_final() { return (nsys > 0 && (ens_passes() & 1)) ? vals_b.p : vals_a.p; }
+
+  // Turn this engine into an ensemble of ns systems of counts[s] particles
+  // stored back to back.  Walk items: 64-target blocks, largest systems
+  // first (the persistent walk then ends on the cheap items).
+  void set_ensemble(const int* counts, int ns) {
+    if (ns < 1 || ns > ens_max_systems())
+      throw Error(kErrInvalid, "system count must be in [1, " +
+                                   std::to_string(ens_max_systems()) + "]");
+    std::vector<int> off(ns + 1, 0);
+    for (int s = 0; s < ns; ++s) {
+      if (counts[s] < 2) throw Error(kErrInvalid, "every system needs at least 2 particles");
+      off[s + 1] = off[s] + counts[s];
+    }
+    if (off[ns] != n) throw Error(kErrInvalid, "system counts do not add up to the engine size");
+    std::vector<int> order(ns);
+    for (int s = 0; s < ns; ++s) order[s] = s;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return counts[a] > counts[b]; });
+    std::vector<int4> it;
+    for (int s : order)
+      for (int t = off[s]; t < off[s + 1]; t += 64)
+        it.push_back(make_int4(s, t, std::min(64, off[s + 1] - t), 0));
+    std::vector<int2> tl;
+    std::vector<int> toff(ns + 1, 0);
+    for (int s = 0; s < ns; ++s) {
+      for (int t = off[s]; t < off[s + 1]; t += kBlock) tl.push_back(make_int2(s, t));
+      toff[s + 1] = (int)tl.size();
+    }
+    drop_graph();
+    release_bh();
+    sys_off.alloc(ns + 1);
+    NB_HIP(hipMemcpy(sys_off.p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+    items.alloc(it.size());
+    NB_HIP(hipMemcpy(items.p, it.data(), it.size() * sizeof(int4), hipMemcpyHostToDevice));
+    e_tiles.alloc(tl.size());
+    NB_HIP(hipMemcpy(e_tiles.p, tl.data(), tl.size() * sizeof(int2), hipMemcpyHostToDevice));
+    e_tile_off.alloc(ns + 1);
+    NB_HIP(hipMemcpy(e_tile_off.p, toff.data(), toff.size() * sizeof(int), hipMemcpyHostToDevice));
+    roots.alloc(ns, true);
+    sys_box.alloc((size_t)8 * ns);
+    nsys = ns;
+    nitems = (int)it.size();
+    ntiles_e = (int)tl.size();
+    h_off = off;
+    h_items = it;
+    Params q = p;
+    q.force = 1;
+    q.traverse = 1;
+    q.group_width = 64;
+    q.key_bits = 30;
+    p = q;
+  }
+
+  // (total, kinetic, potential) of every system, FP64.
+  void system_energy(double* out) {
+    if (nsys == 0) throw Error(kErrState, "not an ensemble");
+    if ((int)en_partial.count < 2 * ntiles_e) en_partial.alloc((size_t)2 * ntiles_e);
+    if ((int)en_out.count < 3 * nsys) en_out.alloc((size_t)3 * nsys);
+    launch_ens_energy(stream, pos, vel, sys_off.p, e_tiles.p, ntiles_e, e_tile_off.p, nsys,
+                      p.eps * p.eps, p.G, en_partial.p, en_out.p);
+    NB_HIP(hipMemcpyAsync(out, en_out.p, 3 * (size_t)nsys * sizeof(double),
+                          hipMemcpyDeviceToHost, stream));
+    NB_HIP(hipStreamSynchronize(stream));
+  }
+
   // Allocations a force pass needs, done before (never during) graph capture.
   void prepare() {
-    if (p.force == 1 && n >= 2) ensure_bh();
+    if ((p.force == 1 && n >= 2) || nsys > 0) ensure_bh();
     if (p.force == 0) {
       const int split = allpairs_split(n);
       if (split > 1 && ap_partial.count < (size_t)split * n) ap_partial.alloc((size_t)split * n);
@@ -267,6 +388,14 @@ class Engine {
 
   void energy(double out[3]) {
     if (n < 1) throw Error(kErrState, "no particles");
+    if (nsys > 0) {  // no energy between systems
+      std::vector<double> e((size_t)3 * nsys);
+      system_energy(e.data());
+      out[0] = out[1] = out[2] = 0.0;
+      for (int s = 0; s < nsys; ++s)
+        for (int k = 0; k < 3; ++k) out[k] += e[3 * s + k];
+      return;
+    }
     int np = energy_partials(n);
     if ((int)en_partial.count < 2 * np) en_partial.alloc((size_t)2 * np);
     if (en_out.count < 3) en_out.alloc(3);
@@ -299,6 +428,7 @@ class Engine {
     // the sort state is reset by each step's Karras kernel for the next step;
     // start from a reset state on this stream
     launch_sort_init(stream, state.p, kMaxPasses);
+    if (nsys > 0) launch_ens_box_init(stream, sys_box.p, nsys);
     bh_cap = n;
   }
   void release_bh() {
@@ -398,6 +528,14 @@ class Engine {
   DevBuf<QuantParams> quant;
   DevBuf<SortState> state;
   DevBuf<unsigned long long> lookback;
+  // ensemble (nsys > 0)
+  int nsys = 0, nitems = 0, ntiles_e = 0;
+  std::vector<int> h_off;
+  std::vector<int4> h_items;
+  DevBuf<int> sys_off, roots, e_tile_off;
+  DevBuf<unsigned> sys_box;
+  DevBuf<int4> items;
+  DevBuf<int2> e_tiles;
 
  private:
   std::vector<hipEvent_t> free_events_;
@@ -487,6 +625,69 @@ int nb_create(int n, nb_engine** out) {
     auto h = std::make_unique<nb_engine>();
     h->e = std::make_unique<nb::Engine>(n);
     *out = h.release();
+  });
+}
+
+int nb_create_ensemble(int nsys, const int* counts, nb_engine** out) {
+  return guarded([&] {
+    need(out, "out");
+    need(counts, "counts");
+    *out = nullptr;
+    if (nsys < 1) throw nb::Error(nb::kErrInvalid, "nsys must be >= 1");
+    long long total = 0;
+    for (int s = 0; s < nsys; ++s) total += counts[s];
+    if (total >= (1ll << 30)) throw nb::Error(nb::kErrInvalid, "ensemble too large (>= 2^30)");
+    auto h = std::make_unique<nb_engine>();
+    h->e = std::make_unique<nb::Engine>((int)std::max(total, 1ll));
+    h->e->set_ensemble(counts, nsys);
+    *out = h.release();
+  });
+}
+
+int nb_ensemble_systems(const nb_engine* h) { return h ? h->e->nsys : -1; }
+
+int nb_ensemble_offsets(const nb_engine* h, int32_t* offsets) {
+  return guarded([&] {
+    need(h, "engine");
+    need(offsets, "offsets");
+    if (h->e->nsys == 0) throw nb::Error(nb::kErrState, "not an ensemble");
+    std::copy(h->e->h_off.begin(), h->e->h_off.end(), offsets);
+  });
+}
+
+int nb_ensemble_items(const nb_engine* h, int32_t* items4, int* nitems) {
+  return guarded([&] {
+    need(h, "engine");
+    if (h->e->nsys == 0) throw nb::Error(nb::kErrState, "not an ensemble");
+    if (nitems) *nitems = h->e->nitems;
+    if (items4)
+      for (size_t k = 0; k < h->e->h_items.size(); ++k) {
+        const int4 v = h->e->h_items[k];
+        items4[4 * k] = v.x;
+        items4[4 * k + 1] = v.y;
+        items4[4 * k + 2] = v.z;
+        items4[4 * k + 3] = v.w;
+      }
+  });
+}
+
+int nb_ensemble_roots(nb_engine* h, int32_t* roots) {
+  return guarded([&] {
+    need(h, "engine");
+    need(roots, "roots");
+    nb::Engine& e = *h->e;
+    if (e.nsys == 0) throw nb::Error(nb::kErrState, "not an ensemble");
+    if (!e.tree_valid) throw nb::Error(nb::kErrState, "no Barnes-Hut tree has been built");
+    e.sync();
+    NB_HIP(hipMemcpy(roots, e.roots.p, e.nsys * sizeof(int), hipMemcpyDeviceToHost));
+  });
+}
+
+int nb_system_energy(nb_engine* h, double* out) {
+  return guarded([&] {
+    need(h, "engine");
+    need(out, "out");
+    h->e->system_energy(out);
   });
 }
 
@@ -680,15 +881,17 @@ int nb_get_tree(nb_engine* h, uint64_t* keys_sorted, uint32_t* perm, int32_t* ch
     e.sync();
     const int n = e.n;
     if (keys_sorted) {
-      if (e.p.key_bits == 30) {
+      if (e.p.key_bits == 30 && e.nsys == 0) {
         std::vector<uint32_t> k(n);
         NB_HIP(hipMemcpy(k.data(), e.keys_a.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
         for (int i = 0; i < n; ++i) keys_sorted[i] = k[i];
       } else {
-        NB_HIP(hipMemcpy(keys_sorted, e.keys_a.p, n * sizeof(uint64_t), hipMemcpyDeviceToHost));
+        NB_HIP(hipMemcpy(keys_sorted, e.keys_final(), n * sizeof(uint64_t),
+                         hipMemcpyDeviceToHost));
       }
     }
-    if (perm) NB_HIP(hipMemcpy(perm, e.vals_a.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (perm)
+      NB_HIP(hipMemcpy(perm, e.perm_final(), n * sizeof(uint32_t), hipMemcpyDeviceToHost));
     if (parent) {
       std::vector<int4> u(2 * (size_t)n - 1);
       NB_HIP(hipMemcpy(u.data(), e.up.p, u.size() * sizeof(int4), hipMemcpyDeviceToHost));
@@ -727,23 +930,25 @@ int nb_traverse_counts(nb_engine* h, int traverse, int group_width, int32_t* int
     if (traverse == 1 && group_width != 16 && group_width != 32 && group_width != 64)
       throw nb::Error(nb::kErrInvalid, "group_width must be 16, 32 or 64");
     const int n = e.n;
-    const int slots = traverse == 0 ? n : (n + group_width - 1) / group_width;
+    if (e.nsys > 0 && (traverse != 1 || group_width != 64))
+      throw nb::Error(nb::kErrInvalid, "an ensemble counts the wave64 walk only");
+    const int slots = e.nsys > 0 ? e.nitems
+                      : traverse == 0 ? n
+                                      : (n + group_width - 1) / group_width;
     nb::DevBuf<int> ci, co;
     ci.alloc(slots, true);
     co.alloc(slots, true);
-    nb::WalkArgs w{};
-    w.pos_sorted = e.pos_sorted.p;
-    w.perm = e.vals_a.p;
-    w.n = n;
-    w.nodes = e.nodes.p;
-    w.theta2 = e.p.theta * e.p.theta;
-    w.eps2 = e.p.eps * e.p.eps;
-    w.G = e.p.G;
+    nb::WalkArgs w = e.walk_args();
+    w.acc = nullptr;
+    w.compensated = false;
     w.counts = ci.p;
     w.opens = co.p;
-    w.overflow = &e.state.p->overflow;
-    w.error = &e.state.p->error;
-    nb::launch_walk(e.stream, traverse, group_width, w);
+    if (e.nsys > 0) {
+      nb::launch_fill_u32(e.stream, &e.state.p->queue, 0u, 1);
+      nb::launch_walk_items(e.stream, w, e.items.p, e.nitems, e.roots.p, &e.state.p->queue, 0);
+    } else {
+      nb::launch_walk(e.stream, traverse, group_width, w);
+    }
     e.sync();
     NB_HIP(hipMemcpy(interactions, ci.p, slots * sizeof(int), hipMemcpyDeviceToHost));
     if (opens) NB_HIP(hipMemcpy(opens, co.p, slots * sizeof(int), hipMemcpyDeviceToHost));

@@ -143,8 +143,10 @@ __global__ __launch_bounds__(kBlock) void energy_kernel(const float4* __restrict
     for (int k = 0; k < kBlock; ++k) {
       float4 src = tile[k];
       float dx = src.x - bi.x, dy = src.y - bi.y, dz = src.z - bi.z;
-      float m = (t + k == i) ? 0.f : src.w;
-      p = fmaf(m, rsq(dist2(dx, dy, dz) + eps2), p);
+      // self and padding terms are skipped by a select, not weighted by 0
+      // (0 * rsq(0) would be NaN with eps = 0)
+      const float term = fmaf(src.w, rsq(dist2(dx, dy, dz) + eps2), p);
+      p = (t + k == i || t + k >= s1) ? p : term;
     }
     phi += (double)p;
     __syncthreads();

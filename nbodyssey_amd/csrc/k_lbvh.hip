@@ -31,23 +31,6 @@
 namespace nb {
 namespace {
 
-__device__ __forceinline__ uint64_t spread21(uint64_t v) {
-  v &= 0x1fffffULL;
-  v = (v | v << 32) & 0x1f00000000ffffULL;
-  v = (v | v << 16) & 0x1f0000ff0000ffULL;
-  v = (v | v << 8) & 0x100f00f00f00f00fULL;
-  v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
-  v = (v | v << 2) & 0x1249249249249249ULL;
-  return v;
-}
-
-__device__ __forceinline__ uint32_t quant21(float p, double lo, double inv) {
-  double v = ((double)p - lo) * inv * 2097152.0;
-  v = (v > 0.0) ? v : 0.0;
-  v = (v > 2097151.0) ? 2097151.0 : v;
-  return (uint32_t)v;
-}
-
 template <typename K>
 __device__ __forceinline__ K make_key(float4 p, double lx, double ly, double lz, double inv) {
   uint64_t k = (spread21(quant21(p.x, lx, inv)) << 2) | (spread21(quant21(p.y, ly, inv)) << 1) |
@@ -665,6 +648,16 @@ void launch_sort(hipStream_t s, int key_bits, void* keys_a, void* keys_b, unsign
   else
     sort_typed<uint64_t>(s, (uint64_t*)keys_a, (uint64_t*)keys_b, vals_a, vals_b, n, ss, lookback,
                          max_tiles, first_pass_identity, gather_src, gather_dst, 8);
+}
+
+void launch_sort_u64(hipStream_t s, uint64_t* keys_a, uint64_t* keys_b, unsigned* vals_a,
+                     unsigned* vals_b, int n, int passes, SortState* ss,
+                     unsigned long long* lookback, int max_tiles, bool first_pass_identity,
+                     const float4* gather_src, float4* gather_dst) {
+  if (n <= 0) return;
+  if (passes < 1 || passes > kMaxPasses) throw Error(kErrInvalid, "sort pass count out of range");
+  sort_typed<uint64_t>(s, keys_a, keys_b, vals_a, vals_b, n, ss, lookback, max_tiles,
+                       first_pass_identity, gather_src, gather_dst, passes);
 }
 
 void launch_karras(hipStream_t s, int key_bits, const void* keys, int n, Node* nodes, int4* up,

@@ -24,6 +24,8 @@
 #include "nb_device.hpp"
 #include "nb_internal.hpp"
 
+#include <algorithm>
+
 namespace nb {
 namespace {
 
@@ -140,25 +142,22 @@ __device__ __forceinline__ void flush_list(const float4* list, int cnt, float bx
   }
 }
 
+// One group of `count` (<= 64) consecutive sorted targets starting at t0,
+// walking the subtree under `root`, on one wave; `slot` indexes the count
+// outputs.  Dead lanes (lane >= count) repeat the group's last target, so the
+// group box and the representatives are those of the live targets.
 template <bool COUNT, bool COMP>
-__global__ __launch_bounds__(kBlock) void walk_wave_kernel(
-    const float4* __restrict__ pos_sorted, const unsigned* __restrict__ perm, int n,
-    const Node* __restrict__ nodes, float theta2, float eps2, float G, float4* __restrict__ acc,
-    float4* __restrict__ vel, float half_dt, int* __restrict__ counts, int* __restrict__ opens,
-    int* overflow, int* error) {
-  constexpr int WAVES = kBlock / 64;
-  __shared__ int s_stack[WAVES][kWaveStack];
-  __shared__ float4 s_list[WAVES][kWaveList];
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void walk_group64(
+    int t0, int count, int root, int slot, const float4* __restrict__ pos_sorted,
+    const unsigned* __restrict__ perm, int n, const Node* __restrict__ nodes, float theta2,
+    float eps2, float G, float4* __restrict__ acc, float4* __restrict__ vel, float half_dt,
+    int* __restrict__ counts, int* __restrict__ opens, int* overflow, int* error, int* stk,
+    float4* list) {
   const int lane = threadIdx.x & 63;
-  const int w = threadIdx.x >> 6;
-  const int gbase = t - lane;
-  if (gbase >= n) return;  // wave-uniform
-  int* stk = s_stack[w];
-  float4* list = s_list[w];
-  const bool live = t < n;
+  const bool live = lane < count;
+  const int t = t0 + (live ? lane : count - 1);
   const int nm1 = n - 1;
-  const float4 b = pos_sorted[live ? t : nm1];
+  const float4 b = pos_sorted[t];
   const unsigned long long lt = (1ull << lane) - 1ull;
 
   // bounding box of the live targets (dead lanes repeat the last target)
@@ -184,7 +183,7 @@ __global__ __launch_bounds__(kBlock) void walk_wave_kernel(
   long ni = 0, no = 0;
   const unsigned nodes_total = 2u * (unsigned)n - 1u;
   unsigned pops = 0;
-  if (lane == 0) stk[0] = 0;
+  if (lane == 0) stk[0] = root;
   while (sp > 0) {
     // pop up to 64 entries; near the top of the stack fall back to one at a
     // time (a depth-first walk needs at most one entry per tree level, and
@@ -284,10 +283,55 @@ __global__ __launch_bounds__(kBlock) void walk_wave_kernel(
   }
   if (nlist > 0) flush_list<COMP>(list, nlist, b.x, b.y, b.z, eps2, ax, ay, az, cx, cy, cz);
   if (COUNT && lane == 0) {
-    counts[gbase >> 6] = (int)ni;
-    opens[gbase >> 6] = (int)no;
+    counts[slot] = (int)ni;
+    opens[slot] = (int)no;
   }
   if (live) finish(t, n, ax, ay, az, G, perm, acc, vel, half_dt);
+}
+
+template <bool COUNT, bool COMP>
+__global__ __launch_bounds__(kBlock) void walk_wave_kernel(
+    const float4* __restrict__ pos_sorted, const unsigned* __restrict__ perm, int n,
+    const Node* __restrict__ nodes, float theta2, float eps2, float G, float4* __restrict__ acc,
+    float4* __restrict__ vel, float half_dt, int* __restrict__ counts, int* __restrict__ opens,
+    int* overflow, int* error) {
+  constexpr int WAVES = kBlock / 64;
+  __shared__ int s_stack[WAVES][kWaveStack];
+  __shared__ float4 s_list[WAVES][kWaveList];
+  const int w = threadIdx.x >> 6;
+  const int gbase = blockIdx.x * blockDim.x + (threadIdx.x & ~63);
+  if (gbase >= n) return;  // wave-uniform
+  walk_group64<COUNT, COMP>(gbase, min(64, n - gbase), 0, gbase >> 6, pos_sorted, perm, n, nodes,
+                            theta2, eps2, G, acc, vel, half_dt, counts, opens, overflow, error,
+                            s_stack[w], s_list[w]);
+}
+
+// Persistent ensemble walk: each wave pulls work items (system, first sorted
+// target, count <= 64) from a device queue until the queue is empty, walking
+// the system's own subtree (root = roots[system]) of the forest.  Every wave
+// exits once the counter passes the item count.
+template <bool COUNT, bool COMP>
+__global__ __launch_bounds__(kBlock) void walk_items_kernel(
+    const float4* __restrict__ pos_sorted, const unsigned* __restrict__ perm, int n,
+    const Node* __restrict__ nodes, float theta2, float eps2, float G, float4* __restrict__ acc,
+    float4* __restrict__ vel, float half_dt, int* __restrict__ counts, int* __restrict__ opens,
+    int* overflow, int* error, const int4* __restrict__ items, int nitems,
+    const int* __restrict__ roots, unsigned* queue) {
+  constexpr int WAVES = kBlock / 64;
+  __shared__ int s_stack[WAVES][kWaveStack];
+  __shared__ float4 s_list[WAVES][kWaveList];
+  const int w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  while (true) {
+    unsigned k = 0;
+    if (lane == 0) k = atomic_add_relaxed(queue, 1u);
+    k = __builtin_amdgcn_readfirstlane(k);
+    if (k >= (unsigned)nitems) break;
+    const int4 it = items[k];  // (system, first target, count, -)
+    walk_group64<COUNT, COMP>(it.y, it.z, roots[it.x], (int)k, pos_sorted, perm, n, nodes, theta2,
+                              eps2, G, acc, vel, half_dt, counts, opens, overflow, error,
+                              s_stack[w], s_list[w]);
+  }
 }
 
 template <int W, bool COUNT>
@@ -383,6 +427,39 @@ void walk_dispatch(hipStream_t s, int traverse, int width, const WalkArgs& a) {
 }
 
 }  // namespace
+
+// Resident workgroups of the persistent walk: CUs x occupancy, queried once.
+static int walk_items_grid() {
+  static int grid = 0;
+  if (grid == 0) {
+    int dev = 0, cus = 0, per_cu = 0;
+    NB_HIP(hipGetDevice(&dev));
+    NB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    NB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(walk_items_kernel<false, false>), kBlock, 0));
+    grid = std::max(1, cus) * std::max(1, per_cu);
+  }
+  return grid;
+}
+
+void launch_walk_items(hipStream_t s, const WalkArgs& a, const int4* items, int nitems,
+                       const int* roots, unsigned* queue, int grid) {
+  if (nitems <= 0) return;
+  if (grid <= 0) grid = walk_items_grid();
+  // a workgroup runs 4 waves; more workgroups than items/4 would only idle
+  grid = std::min(grid, (nitems + kBlock / 64 - 1) / (kBlock / 64));
+#define NB_ITEM_ARGS                                                                           \
+  a.pos_sorted, a.perm, a.n, a.nodes, a.theta2, a.eps2, a.G, a.acc, a.vel, a.half_dt, a.counts, \
+      a.opens, a.overflow, a.error, items, nitems, roots, queue
+  if (a.counts)
+    walk_items_kernel<true, false><<<grid, kBlock, 0, s>>>(NB_ITEM_ARGS);
+  else if (a.compensated)
+    walk_items_kernel<false, true><<<grid, kBlock, 0, s>>>(NB_ITEM_ARGS);
+  else
+    walk_items_kernel<false, false><<<grid, kBlock, 0, s>>>(NB_ITEM_ARGS);
+#undef NB_ITEM_ARGS
+  NB_LAUNCH_CHECK();
+}
 
 void launch_walk(hipStream_t s, int traverse, int group_width, const WalkArgs& a) {
   if (a.n <= 0) return;

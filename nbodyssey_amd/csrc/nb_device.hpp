@@ -5,9 +5,32 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+
 namespace nb {
 
 __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// Morton quantisation (forces_bh.cu:192-220), FP64 from the FP64 origin lo
+// and the cubic scale inv = 1 / longest box edge:
+//   q = clamp(trunc(((p - lo) * inv) * 2^21), 0, 2^21 - 1);
+// spread21 interleaves q's 21 bits with two zero bits each.
+__device__ __forceinline__ uint64_t spread21(uint64_t v) {
+  v &= 0x1fffffULL;
+  v = (v | v << 32) & 0x1f00000000ffffULL;
+  v = (v | v << 16) & 0x1f0000ff0000ffULL;
+  v = (v | v << 8) & 0x100f00f00f00f00fULL;
+  v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+  v = (v | v << 2) & 0x1249249249249249ULL;
+  return v;
+}
+
+__device__ __forceinline__ uint32_t quant21(float p, double lo, double inv) {
+  double v = ((double)p - lo) * inv * 2097152.0;
+  v = (v > 0.0) ? v : 0.0;
+  v = (v > 2097151.0) ? 2097151.0 : v;
+  return (uint32_t)v;
+}
 
 // |d|^2 without softening, in the canonical order shared with the oracle.
 __device__ __forceinline__ float dist2(float dx, float dy, float dz) {

@@ -77,7 +77,7 @@ struct SortState {
   unsigned hist[kMaxPasses][256];
   unsigned ticket[kMaxPasses];
   unsigned epoch;
-  unsigned pad0;
+  unsigned queue;           // work-item counter of the persistent ensemble walk
   int overflow;             // traversal stack overflow flag
   int error;                // consistency guard tripped (bad sort/tree data)
   int pad[3];
@@ -124,6 +124,12 @@ void launch_sort(hipStream_t s, int key_bits, void* keys_a, void* keys_b, unsign
                  float4* gather_dst);
 // (the sort sets SortState::error if a tile id or destination is out of range)
 int sort_passes(int key_bits);
+// 64-bit keys sorted on their low 8*passes bits; the result lands in the *_a
+// buffers for an even pass count and in the *_b buffers for an odd one.
+void launch_sort_u64(hipStream_t s, uint64_t* keys_a, uint64_t* keys_b, unsigned* vals_a,
+                     unsigned* vals_b, int n, int passes, SortState* ss,
+                     unsigned long long* lookback, int max_tiles, bool first_pass_identity,
+                     const float4* gather_src, float4* gather_dst);
 // Also resets `ss` (histograms of `passes` passes, tickets, epoch) for the
 // next sort when ss is not null.
 void launch_karras(hipStream_t s, int key_bits, const void* keys, int n, Node* nodes, int4* up,
@@ -150,5 +156,32 @@ struct WalkArgs {
   bool compensated;   // Kahan-add each block of interactions (wave64 walk)
 };
 void launch_walk(hipStream_t s, int traverse, int group_width, const WalkArgs& a);
+// Persistent wave64 walk over ensemble work items (system, first target,
+// count <= 64, 0), each system from its own subtree root; `queue` must be 0.
+void launch_walk_items(hipStream_t s, const WalkArgs& a, const int4* items, int nitems,
+                       const int* roots, unsigned* queue, int grid);
+
+// k_ensemble.hip: systems stored back to back, system s at [off[s], off[s+1]).
+// box: 8 u32 per system (orderable-float min xyz, max xyz, 2 unused), reset
+// to empty by launch_ens_box_init and afterwards by each step's Karras kernel.
+int ens_passes();
+int ens_max_systems();
+void launch_ens_box_init(hipStream_t s, unsigned* box, int nsys);
+void launch_ens_kick_drift_bbox(hipStream_t s, float4* pos, float4* vel, const float4* acc,
+                                int n, float half_dt, float dt, bool kick, const int* off,
+                                int nsys, unsigned* box);
+void launch_ens_morton_hist(hipStream_t s, const float4* pos, int n, const int* off, int nsys,
+                            const unsigned* box, uint64_t* keys, SortState* ss);
+// Also resets the sort state (ens_passes() passes), the walk queue and the
+// boxes for the next step, and writes roots[s] = the internal node whose leaf
+// range is exactly system s.
+void launch_ens_karras(hipStream_t s, const uint64_t* keys, int n, const int* off, int nsys,
+                       Node* nodes, int4* up, unsigned* visit, int* roots, SortState* ss,
+                       unsigned* box);
+// tiles: (system, first target) per 256-target tile; tile_off[s] = first tile
+// of system s; out: (total, kinetic, potential) per system.
+void launch_ens_energy(hipStream_t s, const float4* pos, const float4* vel, const int* off,
+                       const int2* tiles, int ntiles, const int* tile_off, int nsys, float eps2,
+                       float G, double* partial, double* out);
 
 }  // namespace nb

@@ -169,6 +169,64 @@ class Simulation:
         return inter, opens
 
 
+class Ensemble(Simulation):
+    """Independent systems batched in one engine (nb_create_ensemble).
+
+    ``systems`` is a list of (pos4, vel4) arrays; they are stored back to
+    back and stepped together.  Barnes-Hut with the wave64 walk on
+    (system, 30-bit Morton) keys; dt, eps, G, theta apply to every system.
+    """
+
+    def __init__(self, systems, dt=0.01, eps=0.05, G=1.0, theta=0.5, compensated=False):
+        L = _lib.load()
+        pos = [as_f4(p) for p, _ in systems]
+        vel = [np.zeros_like(p) if v is None else as_f4(v, len(p))
+               for p, (_, v) in zip(pos, systems)]
+        self.counts = np.array([len(p) for p in pos], np.int32)
+        self.n = int(self.counts.sum())
+        self.nsys = len(pos)
+        h = ctypes.c_void_p()
+        check(L.nb_create_ensemble(self.nsys, ptr(self.counts), ctypes.byref(h)))
+        self._h = h
+        self._L = L
+        self.offsets = np.zeros(self.nsys + 1, np.int32)
+        check(L.nb_ensemble_offsets(self._h, ptr(self.offsets)))
+        self.set_params(dt=dt, eps=eps, G=G, theta=theta, force="bh", traverse="warp",
+                        key_bits=30, group_width=64, compensated=compensated)
+        self.upload(np.concatenate(pos), np.concatenate(vel))
+
+    def split(self, a):
+        """Per-system views of an (n, ...) array in ensemble order."""
+        return [a[self.offsets[s]:self.offsets[s + 1]] for s in range(self.nsys)]
+
+    def items(self):
+        """Walk work items (system, first sorted target, count, 0) in queue order."""
+        k = ctypes.c_int()
+        check(self._L.nb_ensemble_items(self._h, None, ctypes.byref(k)))
+        out = np.zeros((k.value, 4), np.int32)
+        check(self._L.nb_ensemble_items(self._h, ptr(out), ctypes.byref(k)))
+        return out
+
+    def roots(self):
+        out = np.zeros(self.nsys, np.int32)
+        check(self._L.nb_ensemble_roots(self._h, ptr(out)))
+        return out
+
+    def system_energy(self):
+        """(nsys, 3): total, kinetic, potential of every system (FP64)."""
+        out = np.zeros((self.nsys, 3))
+        check(self._L.nb_system_energy(self._h, ptr(out)))
+        return out
+
+    def traverse_counts(self, traverse="warp", group_width=64):
+        slots = len(self.items())
+        inter = np.zeros(slots, np.int32)
+        opens = np.zeros(slots, np.int32)
+        t = TRAVERSALS[traverse] if isinstance(traverse, str) else int(traverse)
+        check(self._L.nb_traverse_counts(self._h, t, group_width, ptr(inter), ptr(opens)))
+        return inter, opens
+
+
 def morton_keys(pos, key_bits=63):
     """GPU Morton keys of (n,3) or (n,4) positions (lbvh_check.py:45-54)."""
     pos = np.asarray(pos)

@@ -117,6 +117,73 @@ def two_disk_encounter(n_total=12000, seed=0, eps=0.05, r_max=5.0, z_sigma=0.1,
     return pos4, vel4
 
 
+def parabolic_encounter(n_total, seed, mass_ratio, inclination_deg, separation, pericentre,
+                        eps=0.05, r_max=5.0, z_sigma=0.1):
+    """Two disks on a parabolic orbit (ensemble member).
+
+    Disk A has mass 1, disk B mass ``mass_ratio``; particles have equal mass,
+    so B gets round(n q / (1 + q)) of them.  B is tilted by the inclination
+    about x and spins opposite to A.  The centres start ``separation`` apart
+    on the x axis with the parabolic relative velocity of two point masses
+    M = 1 + q whose pericentre is ``pericentre``: |v| = sqrt(2M/d), tangential
+    part sqrt(2 M r_p) / d, the rest radial and closing; both are placed in
+    the centre-of-mass frame.
+    """
+    rng = np.random.default_rng(seed)
+    q = float(mass_ratio)
+    nb = int(round(n_total * q / (1.0 + q)))
+    nb = min(max(nb, 1), n_total - 1)
+    na = n_total - nb
+    pa, va, ma = exponential_disk(na, rng, 1.0, 1.0, r_max, z_sigma, eps, spin=+1.0)
+    pb, vb, mb = exponential_disk(nb, rng, q * 1.0, 1.0, r_max, z_sigma, eps, spin=-1.0)
+    # equal particle masses: (1 + q) / n_total each
+    ma[:] = mb[:] = (1.0 + q) / n_total
+    incl = math.radians(inclination_deg)
+    pb, vb = _rotate_x(pb, incl), _rotate_x(vb, incl)
+    m_tot = 1.0 + q
+    d, rp = float(separation), float(pericentre)
+    v2 = 2.0 * m_tot / d
+    vt = math.sqrt(2.0 * m_tot * rp) / d
+    vr = -math.sqrt(max(v2 - vt * vt, 0.0))
+    r_rel = np.array([d, 0.0, 0.0])
+    v_rel = np.array([vr, vt, 0.0])
+    pa += -(q / m_tot) * r_rel
+    va += -(q / m_tot) * v_rel
+    pb += (1.0 / m_tot) * r_rel
+    vb += (1.0 / m_tot) * v_rel
+    pos = np.concatenate([pa, pb])
+    vel = np.concatenate([va, vb])
+    m = np.concatenate([ma, mb])
+    pos4 = np.column_stack([pos, m]).astype(np.float32)
+    vel4 = np.column_stack([vel, np.zeros(len(m))]).astype(np.float32)
+    return pos4, vel4
+
+
+def ensemble_encounters(nsys=256, seed=0, n_min=2048, n_max=65536, eps=0.05):
+    """Batched-ensemble workload: nsys independent parabolic encounters.
+
+    System k draws its parameters from default_rng(seed + k): particle count
+    log-uniform on [n_min, n_max], mass ratio U[0.25, 1], inclination
+    U[0, 90] degrees, separation U[15, 25], pericentre U[1, 5] (disk scale
+    lengths), then builds ``parabolic_encounter`` with seed
+    (seed + k) * 7919 + 1.  Returns (list of (pos4, vel4), list of dicts).
+    """
+    systems, meta = [], []
+    for k in range(nsys):
+        r = np.random.default_rng(seed + k)
+        n = int(round(math.exp(r.uniform(math.log(n_min), math.log(n_max)))))
+        n = min(max(n, n_min), n_max)
+        prm = dict(n=n, mass_ratio=float(r.uniform(0.25, 1.0)),
+                   inclination_deg=float(r.uniform(0.0, 90.0)),
+                   separation=float(r.uniform(15.0, 25.0)),
+                   pericentre=float(r.uniform(1.0, 5.0)))
+        systems.append(parabolic_encounter(n, (seed + k) * 7919 + 1, prm["mass_ratio"],
+                                           prm["inclination_deg"], prm["separation"],
+                                           prm["pericentre"], eps=eps))
+        meta.append(prm)
+    return systems, meta
+
+
 # Workloads the benchmark and the tests name (sizes, softening, step, theta).
 WORKLOADS = {
     "two-disk-12k-naive": dict(n=12000, seed=0, eps=0.05, dt=0.01, steps=200, theta=0.5,
@@ -125,6 +192,8 @@ WORKLOADS = {
                            force="bh", key_bits=30),
     "two-disk-4M-precision": dict(n=4194304, seed=2, eps=0.02, dt=0.005, steps=500,
                                   theta=0.5, force="bh", key_bits=30),
+    "ensemble-256-parabolic": dict(nsys=256, seed=0, eps=0.05, dt=0.01, steps=200, theta=0.5,
+                                   force="bh"),
 }
 
 

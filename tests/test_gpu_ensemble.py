@@ -1,0 +1,157 @@
+"""GPU parity of the batched ensemble (nb_create_ensemble).
+
+The forest build must give every system exactly its standalone tree (keys,
+order, topology, node values bit-exact against the oracle), and the
+persistent walk over (system, 64-target block) items must reproduce the
+standalone wave64 walk: interaction/opening counts equal to the oracle's
+64-wide group walk, accelerations and steps bitwise equal to running each
+system alone on the plain path.
+"""
+import numpy as np
+import pytest
+
+from nbodyssey_amd import NbError, engine, ic
+from oracle import orc
+
+pytestmark = pytest.mark.gpu
+
+EPS = 0.05
+
+
+def mixed_systems():
+    """Systems of awkward sizes and shapes: 2, 3, 64, 65 particles, coincident
+    points, duplicate-heavy blobs, two-disk encounters."""
+    rng = np.random.default_rng(3)
+
+    def cloud(n, seed):
+        r = np.random.default_rng(seed)
+        p = r.normal(0.0, 1.0, size=(n, 3))
+        return np.column_stack([p, np.full(n, 1.0 / n)]).astype(np.float32)
+
+    same = np.tile(np.array([[0.3, 0.3, 0.3, 0.01]], np.float32), (777, 1))
+    blobs = np.concatenate([cloud(1500, 1), cloud(1500, 2) + np.float32([4, 0.5, 0.1, 0]),
+                            np.tile(np.float32([[0.123, -0.456, 0.789, 1.0 / 3024]]), (24, 1))])
+    out = [cloud(2, 5), cloud(3, 6), cloud(64, 7), cloud(65, 8), same, blobs,
+           ic.two_disk_encounter(5000, seed=7)[0], cloud(130, 9),
+           ic.parabolic_encounter(4096, 11, 0.5, 45.0, 18.0, 2.0)[0]]
+    order = rng.permutation(len(out))
+    systems = []
+    for k in order:
+        p = out[k]
+        v = rng.normal(0.0, 0.1, size=(len(p), 4)).astype(np.float32)
+        v[:, 3] = 0
+        systems.append((p, v))
+    return systems
+
+
+def local_to_forest(c, m, o, n, root):
+    """Standalone node id c (m particles) -> forest id (system at offset o,
+    n particles in the ensemble, subtree root `root`)."""
+    c = np.asarray(c, np.int64)
+    return np.where(c >= m - 1, (n - 1) + o + (c - (m - 1)), np.where(c == 0, root, o + c))
+
+
+def test_forest_matches_standalone_trees():
+    systems = mixed_systems()
+    with engine.Ensemble(systems, eps=EPS, theta=0.5) as ens:
+        ens.compute_forces()
+        t = ens.tree()
+        roots = ens.roots()
+        off = ens.offsets
+    n = int(off[-1])
+    print(f"forest n={n} systems={len(systems)} roots={roots.tolist()}", flush=True)
+    for s, (pos, _) in enumerate(systems):
+        o, m = int(off[s]), len(pos)
+        ref = orc.Tree(pos, key_bits=30)
+        k = t["keys"][o:o + m]
+        assert np.all((k >> np.uint64(30)) == s)
+        assert np.array_equal(k & np.uint64((1 << 30) - 1), ref.keys_sorted)
+        assert np.array_equal(t["perm"][o:o + m].astype(np.int64) - o, ref.perm)
+        assert np.array_equal(t["pos_sorted"][o:o + m], ref.pos_sorted)
+        g = local_to_forest(np.arange(m - 1), m, o, n, roots[s])
+        assert np.array_equal(t["children"][g], local_to_forest(ref.children, m, o, n, roots[s]))
+        assert np.array_equal(t["com4"][g], ref.com4)
+        assert np.array_equal(t["size2"][g], ref.size2)
+        # every node of the subtree but its root has its standalone parent
+        nodes = np.arange(1, 2 * m - 1)
+        assert np.array_equal(t["parent"][local_to_forest(nodes, m, o, n, roots[s])],
+                              local_to_forest(ref.parent[1:], m, o, n, roots[s]))
+
+
+@pytest.mark.parametrize("theta", [0.0, 0.5, 0.8])
+def test_item_walk_counts_and_forces(theta):
+    """Per-item counts equal the oracle's 64-wide group walk of each system
+    alone; accelerations are bitwise those of the standalone wave64 walk."""
+    systems = mixed_systems()
+    with engine.Ensemble(systems, eps=EPS, theta=theta) as ens:
+        acc = ens.compute_forces()
+        inter, opens = ens.traverse_counts()
+        items = ens.items()
+        off = ens.offsets
+    print(f"ensemble walk theta={theta} items={len(items)}", flush=True)
+    assert np.array_equal(np.sort(items[:, 1]), np.sort(np.concatenate(
+        [np.arange(off[s], off[s + 1], 64) for s in range(len(systems))])))
+    for s, (pos, _) in enumerate(systems):
+        o = int(off[s])
+        ref = orc.Tree(pos, key_bits=30)
+        _, _, ri, ro, _ = ref.traverse_group(theta, EPS, 64)
+        mine = items[:, 0] == s
+        grp = (items[mine, 1] - o) // 64
+        assert np.array_equal(inter[mine], ri[grp])
+        assert np.array_equal(opens[mine], ro[grp])
+        with engine.Simulation(pos, eps=EPS, theta=theta, force="bh", traverse="warp",
+                               group_width=64, key_bits=30) as sim:
+            a1 = sim.compute_forces()
+        assert np.array_equal(acc[o:o + len(pos)], a1), f"system {s} forces differ"
+
+
+def test_steps_and_energy_match_standalone():
+    """Three KDK steps of a 24-system ensemble (sizes 2,048..16,384) equal the
+    systems stepped one at a time, bitwise; per-system FP64 energies match
+    the oracle (rtol 2e-6) and nb_total_energy is their sum."""
+    systems, _ = ic.ensemble_encounters(24, seed=100, n_min=2048, n_max=16384)
+    kw = dict(dt=0.01, eps=EPS, theta=0.5)
+    with engine.Ensemble(systems, **kw) as ens:
+        e_sys = ens.system_energy()
+        e_tot = ens.total_energy()
+        ens.step(3)
+        p_e, v_e = ens.download()
+        off = ens.offsets
+    np.testing.assert_allclose(e_tot, e_sys.sum(axis=0), rtol=1e-12)
+    for s, (pos, vel) in enumerate(systems):
+        r = orc.energy_f64(pos, vel, EPS)
+        np.testing.assert_allclose(e_sys[s], r, rtol=2e-6, atol=1e-12)
+        with engine.Simulation(pos, vel, force="bh", traverse="warp", group_width=64,
+                               key_bits=30, **kw) as sim:
+            sim.step(3)
+            p1, v1 = sim.download()
+        o, m = int(off[s]), len(pos)
+        assert np.array_equal(p_e[o:o + m], p1) and np.array_equal(v_e[o:o + m], v1), s
+
+
+def test_ensemble_graph_matches_eager():
+    systems, _ = ic.ensemble_encounters(16, seed=200, n_min=2048, n_max=8192)
+    kw = dict(dt=0.01, eps=EPS, theta=0.5)
+    with engine.Ensemble(systems, **kw) as eager, engine.Ensemble(systems, **kw) as graph:
+        graph.set_graph(True)
+        for r in range(3):
+            print(f"ensemble graph replay {r}", flush=True)
+            eager.step(1)
+            graph.step(1)
+            pe, ve = eager.download()
+            pg, vg = graph.download()
+            assert np.array_equal(pe, pg) and np.array_equal(ve, vg), f"replay {r} differs"
+            assert np.array_equal(eager.system_energy(), graph.system_energy())
+
+
+def test_ensemble_parameter_checks():
+    systems = [(np.zeros((4, 4), np.float32) + np.float32([[0], [1], [2], [3]]), None)] * 2
+    with engine.Ensemble(systems) as ens:
+        with pytest.raises(NbError):
+            ens.update(force="naive")
+        with pytest.raises(NbError):
+            ens.update(group_width=32)
+        ens.update(theta=0.7, compensated=True)
+        ens.compute_forces()
+    with pytest.raises(NbError):
+        engine.Ensemble([(np.zeros((1, 4), np.float32), None)])

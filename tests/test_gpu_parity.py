@@ -296,3 +296,35 @@ def test_graph_step_matches_eager(force):
         eager.step(2)
         graph.step(2)
         assert np.array_equal(eager.download()[0], graph.download()[0])
+
+
+def test_galaxy_sim_end_to_end(tmp_path):
+    """The reference CLI on the GPU: energy CSV and frames in the reference
+    formats, force report lines, and the --compare-forces gate (main.cu)."""
+    import subprocess
+    from nbodyssey_amd import GALAXY_SIM, formats
+    pos, vel = ic.generate_ic_disks(particles=4000, seed=0)
+    icp = tmp_path / "ic.bin"
+    formats.write_ic(icp, pos, vel)
+    out = tmp_path / "frames"
+    elog = tmp_path / "e" / "energy.csv"
+    r = subprocess.run([GALAXY_SIM, "--ic", str(icp), "--steps", "20", "--dump-every", "5",
+                        "--out", str(out), "--energy-log", str(elog), "--force", "bh",
+                        "--traverse", "warp", "--theta", "0.5"], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    assert "force pipeline: 21 calls" in r.stdout   # a(x0) + 20 steps (main.cu:184)
+    assert "barnes-hut pipeline, per-call average over 21 calls:" in r.stdout
+    e = formats.read_energy_csv(elog)
+    assert list(e["step"]) == [0, 5, 10, 15, 20]
+    assert np.all(np.abs(e["rel_error"]) < 1e-3)
+    frames = sorted(p.name for p in out.iterdir())
+    assert frames == [f"frame_{k:05d}.bin" for k in range(4)]
+    assert formats.read_frame(out / "frame_00000.bin").shape == (4000, 4)
+    g = subprocess.run([GALAXY_SIM, "--ic", str(icp), "--compare-forces", "--theta", "0",
+                        "--traverse", "warp"], capture_output=True, text=True, timeout=120)
+    assert g.returncode == 0
+    assert "naive vs barnes-hut, theta=0, traverse=warp, n=4000" in g.stdout
+    mean = float(g.stdout.strip().split("mean ")[-1])
+    assert mean < 1e-4
