@@ -13,8 +13,8 @@ Prints one JSON line (the metric is particle-steps/s) with:
                 device), divided by the walk kernel's mean duration measured
                 with HIP events on the engine stream during the timed steps;
   accuracy      median / p99 relative force error of the final state's
-                accelerations against an FP64 all-pairs oracle on sampled
-                targets;
+                accelerations against FP64 direct summation on the GPU
+                (nb_direct_f64) for sampled targets;
   cpu_baseline  the reference CPU algorithm (NumPy chunked float32 all-pairs
                 + KDK, scripts/reference_nbody.py:45-61,130-136 restated)
                 timed on this host on a bounded sample and extrapolated.
@@ -166,15 +166,8 @@ def ensemble_main(args):
             ti, to = sim.traverse_counts("thread")
             inter_sum += ti.astype(np.float64).sum()
             opens_sum += to.astype(np.float64).sum()
-            try:
-                from oracle import orc
-                idx = rng.choice(m, per_sys, replace=False).astype(np.int32)
-                ref = orc.allpairs_f64(p_b[o:o + m], eps, targets=idx)
-                err.append(np.linalg.norm(a_s[idx, :3] - ref, axis=1) /
-                           np.linalg.norm(ref, axis=1))
-            except Exception as exc:  # the oracle is a checker only
-                err = None
-                print(f"bench: accuracy check skipped: {exc}", file=sys.stderr)
+            idx = rng.choice(m, per_sys, replace=False).astype(np.int32)
+            err.append(sim.force_error(idx, a_s))
             sim.set_graph(True)
             sim.step(args.serial_warmup)
             sim.synchronize()
@@ -222,6 +215,64 @@ def ensemble_main(args):
     print(json.dumps(out))
 
 
+def precision_main(args):
+    """Precision study on one two-disk system (default 4,194,304 particles,
+    eps 0.02, dt 0.005): for theta in {0.3, 0.5, 0.7} and the FP32 and
+    compensated (Kahan) walks, the step time, the force error on a seeded
+    subsample against FP64 direct summation on the GPU, and the relative
+    total-energy drift over `drift_steps` steps (FP64 energy on the GPU)."""
+    from nbodyssey_amd import engine, ic
+    w = ic.WORKLOADS["two-disk-4M-precision"]
+    n = args.n if "--n" in sys.argv else w["n"]
+    eps, dt = w["eps"], w["dt"]
+    pos, vel = ic.two_disk_encounter(n, seed=w["seed"], eps=eps)
+    idx = np.random.default_rng(0).choice(n, min(args.error_targets, n),
+                                          replace=False).astype(np.int32)
+    rows = []
+    headline = None
+    for theta in (0.3, 0.5, 0.7):
+        for comp in (False, True):
+            with engine.Simulation(pos, vel, dt=dt, eps=eps, theta=theta, force="bh",
+                                   traverse="warp", group_width=64, key_bits=30,
+                                   compensated=comp) as sim:
+                a = sim.compute_forces()
+                err = sim.force_error(idx, a)
+                sim.step(args.warmup)
+                ms = sim.time_steps(args.steps)
+                e0 = None
+                drift = None
+                if args.drift_steps > 0:
+                    sim.upload(pos, vel)
+                    sim.compute_forces()
+                    e0 = sim.total_energy()
+                    sim.step(args.drift_steps)
+                    e1 = sim.total_energy()
+                    drift = float((e1[0] - e0[0]) / abs(e0[0]))
+            row = {"theta": theta, "precision": "kahan" if comp else "fp32",
+                   "ms_per_step": ms / args.steps,
+                   "median_rel_force_err_vs_fp64": float(np.median(err)),
+                   "p99_rel_force_err_vs_fp64": float(np.quantile(err, 0.99)),
+                   "energy_drift": drift, "drift_steps": args.drift_steps}
+            rows.append(row)
+            print(f"bench: {row}", file=sys.stderr, flush=True)
+            if theta == 0.5 and not comp:
+                headline = row
+    value = n / (headline["ms_per_step"] * 1e-3)
+    out = {
+        "metric": f"particle-steps/s @{n} BH theta=0.5 incl. rebuild (precision study)",
+        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": headline["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic two-disk encounter (seeded generator)",
+        "config": {"workload": f"two-disk {n} particles, Barnes-Hut rebuilt every step, 30-bit "
+                               f"keys, wave64 walk, eps={eps}, dt={dt}; theta x precision sweep",
+                   "n": n, "error_targets": int(len(idx))},
+        "precision_study": rows,
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -240,7 +291,10 @@ def main():
     ap.add_argument("--accuracy-targets", type=int, default=256)
     ap.add_argument("--graph", action="store_true",
                     help="replay each step as a captured HIP graph (no per-phase timing)")
-    ap.add_argument("--workload", default="two-disk", choices=["two-disk", "ensemble"],
+    ap.add_argument("--error-targets", type=int, default=65536)
+    ap.add_argument("--drift-steps", type=int, default=500)
+    ap.add_argument("--workload", default="two-disk",
+                    choices=["two-disk", "ensemble", "precision"],
                     help="ensemble: batched independent systems vs the same systems one at a "
                          "time (defaults: 256 systems, 200 steps)")
     ap.add_argument("--systems", type=int, default=256)
@@ -258,6 +312,10 @@ def main():
         if "--steps" not in sys.argv:
             args.steps = 200
         return ensemble_main(args)
+    if args.workload == "precision":
+        if "--steps" not in sys.argv:
+            args.steps = 20
+        return precision_main(args)
 
     from nbodyssey_amd import engine, ic
     eps, dt = 0.02, 0.005
@@ -299,21 +357,14 @@ def main():
     width = 1 if args.traverse == "thread" else args.group_width
     lane_inter = ginter.astype(np.float64).sum() * width if width > 1 else ginter.sum()
 
-    # accuracy of the final state's forces vs FP64 all-pairs on sampled targets
-    acc = sim.acc()
-    p_final, _ = sim.download()
-    accuracy = None
-    try:
-        from oracle import orc
-        idx = np.random.default_rng(0).choice(args.n, args.accuracy_targets,
-                                              replace=False).astype(np.int32)
-        ref = orc.allpairs_f64(p_final, eps, targets=idx)
-        err = np.linalg.norm(acc[idx, :3] - ref, axis=1) / np.linalg.norm(ref, axis=1)
-        accuracy = {"median_rel_force_err_vs_fp64": float(np.median(err)),
-                    "p99_rel_force_err_vs_fp64": float(np.quantile(err, 0.99)),
-                    "targets": int(args.accuracy_targets)}
-    except Exception as exc:  # the oracle is a checker; report, do not fail the bench
-        accuracy = {"error": str(exc)}
+    # accuracy of the final state's forces vs FP64 direct summation on the GPU
+    idx = np.random.default_rng(0).choice(args.n, args.accuracy_targets,
+                                          replace=False).astype(np.int32)
+    err = sim.force_error(idx)
+    accuracy = {"median_rel_force_err_vs_fp64": float(np.median(err)),
+                "p99_rel_force_err_vs_fp64": float(np.quantile(err, 0.99)),
+                "targets": int(args.accuracy_targets),
+                "reference": "FP64 direct summation on the GPU (nb_direct_f64)"}
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -157,6 +157,15 @@ int nb_sort_pairs(const uint64_t* keys, int n, int key_bits, uint64_t* keys_sort
 int nb_karras_build(const uint64_t* keys_sorted, int n, int key_bits, int32_t* children,
                     int32_t* parent);
 
+/* FP64 direct-summation accelerations of nt targets (original particle
+ * indices) from the engine's current positions: 3 doubles per target,
+ * G * sum_{j != i} m_j (r_j - r_i) / (r^2 + eps^2)^{3/2} in FP64 on the
+ * device.  The accuracy reference of the force modules: the FP64 form of
+ * reference_nbody.py:45-61 (accelerations) that the reference's tests and
+ * PERFORMANCE.md:84-92 measure Barnes-Hut error against.  In an ensemble the
+ * sources of a target are its own system. */
+int nb_direct_f64(nb_engine* h, const int32_t* targets, int nt, double* acc3);
+
 /* ---------------------------------------------------------------- ensembles
  * No counterpart in the reference, which runs one system per process
  * (main.cu:72-240); additive, for batched runs of many independent systems.

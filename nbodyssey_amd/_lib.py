@@ -90,6 +90,7 @@ SIGNATURES = {
     "nb_ensemble_items": (_i, [_vp, _vp, ctypes.POINTER(_i)]),
     "nb_ensemble_roots": (_i, [_vp, _vp]),
     "nb_system_energy": (_i, [_vp, _vp]),
+    "nb_direct_f64": (_i, [_vp, _vp, _i, _vp]),
 }
 
 _lib = None

@@ -404,6 +404,57 @@ class Engine {
     NB_HIP(hipStreamSynchronize(stream));
   }
 
+  // FP64 direct-summation accelerations of the given targets (original
+  // indices); an ensemble target's sources are its own system.
+  void direct_f64(const int32_t* targets, int nt, double* acc3) {
+    if (nt <= 0) return;
+    std::vector<int> sys(nt, 0);
+    for (int k = 0; k < nt; ++k) {
+      if (targets[k] < 0 || targets[k] >= n) throw Error(kErrInvalid, "target index out of range");
+      if (nsys > 0)
+        sys[k] = (int)(std::upper_bound(h_off.begin(), h_off.end(), targets[k]) - h_off.begin()) - 1;
+    }
+    const int groups = nsys > 0 ? nsys : 1;
+    std::vector<int> order(nt);
+    for (int k = 0; k < nt; ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sys[a] < sys[b]; });
+    std::vector<int> tsorted(nt);
+    for (int k = 0; k < nt; ++k) tsorted[k] = targets[order[k]];
+    DevBuf<int> dt_;
+    DevBuf<double> part, out;
+    dt_.alloc(nt);
+    NB_HIP(hipMemcpyAsync(dt_.p, tsorted.data(), nt * sizeof(int), hipMemcpyHostToDevice, stream));
+    out.alloc((size_t)3 * nt);
+    size_t need_part = 0;
+    for (int g = 0, k = 0; g < groups && k < nt; ++g) {
+      int k1 = k;
+      while (k1 < nt && sys[order[k1]] == g) ++k1;
+      if (k1 == k) continue;
+      const int lo = nsys > 0 ? h_off[g] : 0, hi = nsys > 0 ? h_off[g + 1] : n;
+      need_part = std::max(need_part, (size_t)direct_f64_split(k1 - k, hi - lo) * (k1 - k) * 3);
+      k = k1;
+    }
+    part.alloc(need_part);
+    const double eps2 = (double)p.eps * (double)p.eps;
+    for (int g = 0, k = 0; g < groups && k < nt; ++g) {
+      int k1 = k;
+      while (k1 < nt && sys[order[k1]] == g) ++k1;
+      if (k1 == k) continue;
+      const int lo = nsys > 0 ? h_off[g] : 0, hi = nsys > 0 ? h_off[g + 1] : n;
+      // one group at a time: the partial buffer is reused
+      launch_direct_f64(stream, pos, lo, hi, dt_.p + k, k1 - k, eps2, (double)p.G, part.p,
+                        out.p + (size_t)3 * k);
+      NB_HIP(hipStreamSynchronize(stream));
+      k = k1;
+    }
+    std::vector<double> res((size_t)3 * nt);
+    NB_HIP(hipMemcpyAsync(res.data(), out.p, res.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          stream));
+    NB_HIP(hipStreamSynchronize(stream));
+    for (int k = 0; k < nt; ++k)
+      for (int c = 0; c < 3; ++c) acc3[(size_t)3 * order[k] + c] = res[(size_t)3 * k + c];
+  }
+
   // ---------------------------------------------------------------- scratch
   void ensure_bh() {
     if (bh_cap >= n) return;
@@ -680,6 +731,18 @@ int nb_ensemble_roots(nb_engine* h, int32_t* roots) {
     if (!e.tree_valid) throw nb::Error(nb::kErrState, "no Barnes-Hut tree has been built");
     e.sync();
     NB_HIP(hipMemcpy(roots, e.roots.p, e.nsys * sizeof(int), hipMemcpyDeviceToHost));
+  });
+}
+
+int nb_direct_f64(nb_engine* h, const int32_t* targets, int nt, double* acc3) {
+  return guarded([&] {
+    need(h, "engine");
+    if (nt < 0) throw nb::Error(nb::kErrInvalid, "nt must be >= 0");
+    if (nt > 0) {
+      need(targets, "targets");
+      need(acc3, "acc3");
+    }
+    h->e->direct_f64(targets, nt, acc3);
   });
 }
 

@@ -11,6 +11,8 @@
 //   * with `compensated`, every tile's FP32 partial is Kahan-added into the
 //     running total (pairwise tile sums);
 //   * the closing kick of the leapfrog step can be fused into the epilogue.
+// direct_f64: FP64 direct summation for chosen targets, the accuracy
+// reference the force error is measured against.
 // energy (integrator.cu:43-110): per-target potential summed in FP32 within a
 // tile and FP64 across tiles, self term excluded exactly, KE and PE reduced in
 // FP64 per block and then in a fixed order by one block (no double atomics).
@@ -209,6 +211,60 @@ __global__ __launch_bounds__(kBlock) void energy_final_kernel(const double* __re
   }
 }
 
+// FP64 direct summation for a list of targets (the accuracy reference of
+// every force module): a_i = sum_{j != i} m_j d / (r2 * sqrt(r2)),
+// r2 = |d|^2 + eps2, all in FP64 from the FP32 state, sources [lo, hi)
+// streamed through LDS; blockIdx.y takes one source chunk and writes its
+// partial sums, added in chunk order by direct_f64_reduce_kernel.
+__global__ __launch_bounds__(kBlock) void direct_f64_kernel(const float4* __restrict__ pos,
+                                                            int lo, int hi, int chunk,
+                                                            const int* __restrict__ targets,
+                                                            int nt, double eps2,
+                                                            double* __restrict__ partial) {
+  __shared__ float4 tile[kBlock];
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  const int i = targets[k < nt ? k : nt - 1];
+  const float4 b = pos[i];
+  const double bx = b.x, by = b.y, bz = b.z;
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  const int s0 = lo + blockIdx.y * chunk;
+  const int s1 = min(hi, s0 + chunk);
+  for (int t = s0; t < s1; t += kBlock) {
+    const int j = t + threadIdx.x;
+    tile[threadIdx.x] = j < s1 ? pos[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+    __syncthreads();
+    const int cnt = min(kBlock, s1 - t);
+    for (int q = 0; q < cnt; ++q) {
+      const float4 src = tile[q];
+      const double dx = (double)src.x - bx, dy = (double)src.y - by, dz = (double)src.z - bz;
+      const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+      const double f = (double)src.w / (r2 * sqrt(r2));
+      if (t + q != i) {
+        ax += dx * f;
+        ay += dy * f;
+        az += dz * f;
+      }
+    }
+    __syncthreads();
+  }
+  if (k < nt) {
+    double* o = partial + ((size_t)blockIdx.y * nt + k) * 3;
+    o[0] = ax;
+    o[1] = ay;
+    o[2] = az;
+  }
+}
+
+__global__ void direct_f64_reduce_kernel(const double* __restrict__ partial, int nt, int split,
+                                         double G, double* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nt) return;
+  double a[3] = {0.0, 0.0, 0.0};
+  for (int y = 0; y < split; ++y)
+    for (int c = 0; c < 3; ++c) a[c] += partial[((size_t)y * nt + k) * 3 + c];
+  for (int c = 0; c < 3; ++c) out[(size_t)k * 3 + c] = G * a[c];
+}
+
 int split_for(int blocks_x, int n) {
   // aim for >= 2048 workgroups (8 per CU) but keep >= 4 tiles per chunk
   int max_split = n / (4 * kBlock);
@@ -252,6 +308,29 @@ void launch_allpairs(hipStream_t s, const float4* pos, int n, float eps2, float 
       split_reduce_kernel<false><<<g, kBlock, 0, s>>>(partial, n, split, G, acc, vel_kick, half_dt);
     NB_LAUNCH_CHECK();
   }
+}
+
+int direct_f64_split(int nt, int nsrc) {
+  const int bx = (nt + kBlock - 1) / kBlock;
+  const int split = split_for(bx, nsrc);
+  const int chunk = chunk_for(nsrc, split);
+  return (nsrc + chunk - 1) / chunk;
+}
+
+void launch_direct_f64(hipStream_t s, const float4* pos, int lo, int hi, const int* targets,
+                       int nt, double eps2, double G, double* partial, double* out) {
+  if (nt <= 0) return;
+  const int nsrc = hi - lo;
+  const int bx = (nt + kBlock - 1) / kBlock;
+  const int split = split_for(bx, nsrc);
+  const int chunk = chunk_for(nsrc, split);
+  const int ny = (nsrc + chunk - 1) / chunk;
+  direct_f64_kernel<<<dim3(bx, ny), kBlock, 0, s>>>(pos, lo, hi, chunk, targets, nt, eps2,
+                                                    partial);
+  NB_LAUNCH_CHECK();
+  direct_f64_reduce_kernel<<<(nt + kBlock - 1) / kBlock, kBlock, 0, s>>>(partial, nt, ny, G,
+                                                                        out);
+  NB_LAUNCH_CHECK();
 }
 
 int energy_partials(int n) {

@@ -104,6 +104,12 @@ void launch_allpairs(hipStream_t s, const float4* pos, int n, float eps2, float 
 void launch_energy(hipStream_t s, const float4* pos, const float4* vel, int n, float eps2,
                    float G, double* partial, int max_partial, double* out);
 int energy_partials(int n);
+// FP64 direct sum for targets[0..nt) (original indices) over sources
+// [lo, hi): out[3k..3k+2] = G * sum_{j != i} m_j d / (r2 sqrt(r2)).
+// partial holds direct_f64_split(nt, hi - lo) * nt * 3 doubles.
+int direct_f64_split(int nt, int nsrc);
+void launch_direct_f64(hipStream_t s, const float4* pos, int lo, int hi, const int* targets,
+                       int nt, double eps2, double G, double* partial, double* out);
 
 // k_lbvh.hip
 // Reduces the bbox partials, forms the FP64 quantisation (block 0 stores it

@@ -119,6 +119,22 @@ class Simulation:
         check(self._L.nb_total_energy(self._h, ptr(out)))
         return out
 
+    def direct_f64(self, targets):
+        """FP64 direct-sum accelerations (nt, 3) of the given original indices."""
+        t = np.ascontiguousarray(targets, dtype=np.int32)
+        out = np.zeros((len(t), 3))
+        check(self._L.nb_direct_f64(self._h, ptr(t), len(t), ptr(out)))
+        return out
+
+    def force_error(self, targets, acc=None):
+        """Relative error |a - a_fp64| / |a_fp64| of this engine's accelerations
+        (the last force evaluation, or `acc`) on the given targets."""
+        t = np.ascontiguousarray(targets, dtype=np.int32)
+        a = self.acc() if acc is None else acc
+        ref = self.direct_f64(t)
+        return np.linalg.norm(a[t, :3].astype(np.float64) - ref, axis=1) / \
+            np.linalg.norm(ref, axis=1)
+
     # -- timing
     def force_ms_total(self):
         return float(self._L.nb_force_ms_total(self._h))

@@ -155,3 +155,19 @@ def test_ensemble_parameter_checks():
         ens.compute_forces()
     with pytest.raises(NbError):
         engine.Ensemble([(np.zeros((1, 4), np.float32), None)])
+
+
+def test_direct_f64_per_system():
+    """In an ensemble the FP64 reference sums over the target's own system
+    only: equal to the oracle on that system alone."""
+    systems = mixed_systems()
+    eps = float(np.float32(EPS))
+    with engine.Ensemble(systems, eps=eps) as ens:
+        off = ens.offsets
+        idx = np.concatenate([np.arange(off[s], min(off[s + 1], off[s] + 40))
+                              for s in range(len(systems))])[::-1].astype(np.int32)
+        got = ens.direct_f64(idx)
+    for s, (pos, _) in enumerate(systems):
+        mine = (idx >= off[s]) & (idx < off[s + 1])
+        ref = orc.allpairs_f64(pos, eps, targets=(idx[mine] - off[s]).astype(np.int32))
+        np.testing.assert_allclose(got[mine], ref, rtol=1e-12, atol=1e-14 * np.abs(ref).max())

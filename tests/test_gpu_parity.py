@@ -328,3 +328,23 @@ def test_galaxy_sim_end_to_end(tmp_path):
     assert "naive vs barnes-hut, theta=0, traverse=warp, n=4000" in g.stdout
     mean = float(g.stdout.strip().split("mean ")[-1])
     assert mean < 1e-4
+
+
+@pytest.mark.parametrize("n", [2, 3, 1000, 65537])
+def test_direct_f64_vs_oracle(n):
+    """nb_direct_f64 (the FP64 accuracy reference on the device) equals the
+    oracle's FP64 direct sum to summation-order round-off (rtol 1e-12), self
+    term excluded; a duplicated point has a finite force."""
+    pos, _ = ic.two_disk_encounter(max(n, 2), seed=21, eps=0.02)
+    pos = pos[:n].copy()
+    if n > 3:
+        pos[5] = pos[4]   # coincident pair
+    eps = float(np.float32(0.02))
+    rng = np.random.default_rng(n)
+    idx = rng.choice(n, min(n, 300), replace=False).astype(np.int32)
+    with engine.Simulation(pos, eps=eps) as sim:
+        got = sim.direct_f64(idx)
+    ref = orc.allpairs_f64(pos, eps, targets=idx)
+    print(f"direct_f64 n={n}", flush=True)
+    assert np.all(np.isfinite(got))
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-14 * np.abs(ref).max())
