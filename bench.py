@@ -131,7 +131,7 @@ def ensemble_main(args):
           f"{time.perf_counter() - t_ic:.1f}s", file=sys.stderr, flush=True)
 
     ens = engine.Ensemble(systems, dt=dt, eps=eps, theta=theta)
-    ens.set_graph(True)
+    ens.set_graph(not args.no_graph)
     ens.compute_forces()
     ens.step(args.warmup)
     ens.synchronize()
@@ -301,6 +301,8 @@ def main():
     ap.add_argument("--serial-steps", type=int, default=20)
     ap.add_argument("--serial-warmup", type=int, default=2)
     ap.add_argument("--phase-steps", type=int, default=10)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="ensemble: time eager steps instead of graph replays")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

@@ -3,8 +3,9 @@
 // forces_naive (forces.cu:33-60): every target sums the softened pull of
 // every source, sources streamed through LDS in kBlock-wide float4 tiles.
 // Differences in this design:
-//   * each thread owns kTargets targets, so one LDS broadcast read feeds
-//     kTargets interactions;
+//   * each thread owns kTargets targets (1: measured fastest on gfx950 --
+//     the loop is VALU-issue bound at ~13 VALU per interaction, and more
+//     targets per thread only raise register pressure);
 //   * for small n the source range is split over blockIdx.y so the grid still
 //     covers all CUs; the partial sums are added in a fixed order by a second
 //     kernel (deterministic, no atomics);
@@ -22,7 +23,7 @@
 namespace nb {
 namespace {
 
-constexpr int kTargets = 2;
+constexpr int kTargets = 1;
 constexpr int kSpan = kBlock * kTargets;   // targets per block
 
 template <bool COMP>
@@ -310,9 +311,19 @@ void launch_allpairs(hipStream_t s, const float4* pos, int n, float eps2, float 
   }
 }
 
+// Source chunks of the FP64 direct sum: enough workgroups to fill the chip
+// (>= 2048) even for a few hundred targets, chunks of >= 4 LDS tiles.
+static int direct_split_for(int bx, int nsrc) {
+  int max_split = nsrc / (4 * kBlock);
+  int split = (2048 + bx - 1) / bx;
+  if (split > max_split) split = max_split;
+  if (split > 4096) split = 4096;
+  return split < 1 ? 1 : split;
+}
+
 int direct_f64_split(int nt, int nsrc) {
   const int bx = (nt + kBlock - 1) / kBlock;
-  const int split = split_for(bx, nsrc);
+  const int split = direct_split_for(bx, nsrc);
   const int chunk = chunk_for(nsrc, split);
   return (nsrc + chunk - 1) / chunk;
 }
@@ -322,7 +333,7 @@ void launch_direct_f64(hipStream_t s, const float4* pos, int lo, int hi, const i
   if (nt <= 0) return;
   const int nsrc = hi - lo;
   const int bx = (nt + kBlock - 1) / kBlock;
-  const int split = split_for(bx, nsrc);
+  const int split = direct_split_for(bx, nsrc);
   const int chunk = chunk_for(nsrc, split);
   const int ny = (nsrc + chunk - 1) / chunk;
   direct_f64_kernel<<<dim3(bx, ny), kBlock, 0, s>>>(pos, lo, hi, chunk, targets, nt, eps2,

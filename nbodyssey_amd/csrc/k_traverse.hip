@@ -29,6 +29,20 @@
 namespace nb {
 namespace {
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Renumber them so that XCD x runs the x-th contiguous eighth of the
+// Morton-ordered targets: neighbouring groups open nearly the same nodes, so
+// each XCD's L2 then holds one region's part of the tree instead of all of it.
+constexpr int kXcds = 8;
+constexpr int kXcdChunk = 16;  // consecutive workgroups kept on one XCD
+__device__ __forceinline__ int xcd_block() {
+  const int b = blockIdx.x, g = gridDim.x;
+  const int span = kXcds * kXcdChunk;
+  if (b >= (g / span) * span) return b;
+  const int x = b % kXcds, k = b / kXcds;
+  return ((k / kXcdChunk) * kXcds + x) * kXcdChunk + k % kXcdChunk;
+}
+
 __device__ __forceinline__ void finish(int t, int n, float ax, float ay, float az, float G,
                                        const unsigned* __restrict__ perm, float4* __restrict__ acc,
                                        float4* __restrict__ vel, float half_dt) {
@@ -51,7 +65,7 @@ __global__ __launch_bounds__(kBlock) void walk_thread_kernel(
     const Node* __restrict__ nodes, float theta2, float eps2, float G, float4* __restrict__ acc,
     float4* __restrict__ vel, float half_dt, int* __restrict__ counts, int* __restrict__ opens,
     int* overflow, int* error) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int nm1 = n - 1;
   const float4 b = pos_sorted[t];
@@ -299,7 +313,7 @@ __global__ __launch_bounds__(kBlock) void walk_wave_kernel(
   __shared__ int s_stack[WAVES][kWaveStack];
   __shared__ float4 s_list[WAVES][kWaveList];
   const int w = threadIdx.x >> 6;
-  const int gbase = blockIdx.x * blockDim.x + (threadIdx.x & ~63);
+  const int gbase = xcd_block() * blockDim.x + (threadIdx.x & ~63);
   if (gbase >= n) return;  // wave-uniform
   walk_group64<COUNT, COMP>(gbase, min(64, n - gbase), 0, gbase >> 6, pos_sorted, perm, n, nodes,
                             theta2, eps2, G, acc, vel, half_dt, counts, opens, overflow, error,
@@ -342,7 +356,7 @@ __global__ __launch_bounds__(kBlock) void walk_group_kernel(
     int* overflow, int* error) {
   constexpr int NG = kBlock / W;
   __shared__ int stk[NG][kStackDepth];
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = xcd_block() * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const int g = threadIdx.x / W;
   const int glane = threadIdx.x % W;
