@@ -70,8 +70,8 @@ def profiled_walk_traffic(args):
     """HBM bytes per walk launch from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary (calibrated against plain streaming kernels), for the
     default workload only; None otherwise."""
-    default = (args.n == 1 << 20 and args.theta == 0.5 and args.traverse == "warp"
-               and args.group_width == 64 and args.key_bits == 30)
+    default = (args.workload == "two-disk" and args.n == 1 << 20 and args.theta == 0.5
+               and args.traverse == "warp" and args.group_width == 64 and args.key_bits == 30)
     path = os.path.join(ROOT, "profiles", "bh_step_1m_theta05_traffic.md")
     if not default or not os.path.exists(path):
         return None
@@ -294,13 +294,16 @@ def main():
     ap.add_argument("--error-targets", type=int, default=65536)
     ap.add_argument("--drift-steps", type=int, default=500)
     ap.add_argument("--workload", default="two-disk",
-                    choices=["two-disk", "ensemble", "precision"],
+                    choices=["two-disk", "ensemble", "precision", "composite"],
                     help="ensemble: batched independent systems vs the same systems one at a "
                          "time (defaults: 256 systems, 200 steps)")
     ap.add_argument("--systems", type=int, default=256)
     ap.add_argument("--serial-steps", type=int, default=20)
     ap.add_argument("--serial-warmup", type=int, default=2)
     ap.add_argument("--phase-steps", type=int, default=10)
+    ap.add_argument("--energy-theta", type=float, default=0.25,
+                    help="composite: theta of the tree-potential energy taken before and after "
+                         "the timed steps (0 = none)")
     ap.add_argument("--no-graph", action="store_true",
                     help="ensemble: time eager steps instead of graph replays")
     args = ap.parse_args()
@@ -319,24 +322,57 @@ def main():
             args.steps = 20
         return precision_main(args)
 
-    from nbodyssey_amd import engine, ic
+    from nbodyssey_amd import ic
+    if args.workload == "composite":
+        w = ic.WORKLOADS["composite-33M-63bit"]
+        n = args.n if "--n" in sys.argv else w["n"]
+        for k in ("theta", "key_bits"):
+            if "--" + k.replace("_", "-") not in sys.argv:
+                setattr(args, k, w[k])
+        t_ic = time.perf_counter()
+        pos, vel = ic.composite_encounter(n, seed=w["seed"], eps=w["eps"])
+        print(f"bench: composite n={n} ic {time.perf_counter() - t_ic:.1f}s", file=sys.stderr,
+              flush=True)
+        args.n = n
+        return bh_main(args, pos, vel, w["eps"], w["dt"],
+                       f"two composite galaxies (disk 60% + Hernquist bulge 10% + Plummer halo "
+                       f"30%) on a parabolic orbit, {n} particles",
+                       f"particle-steps/s @{n} composite BH theta={args.theta} incl. rebuild",
+                       "synthetic composite-galaxy encounter (seeded generator)",
+                       energy_theta=args.energy_theta, cpu=False)
     eps, dt = 0.02, 0.005
     t_ic = time.perf_counter()
     pos, vel = ic.two_disk_encounter(args.n, seed=1, eps=eps)
-    t_ic = time.perf_counter() - t_ic
-    print(f"bench: n={args.n} ic {t_ic:.1f}s", file=sys.stderr, flush=True)
+    print(f"bench: n={args.n} ic {time.perf_counter() - t_ic:.1f}s", file=sys.stderr, flush=True)
+    if args.force == "naive":
+        from nbodyssey_amd import engine
+        sim = engine.Simulation(pos, vel, dt=dt, eps=eps, force="naive")
+        return naive_main(args, sim, eps)
+    return bh_main(args, pos, vel, eps, dt, f"two-disk {args.n} particles",
+                   "particle-steps/s @1M BH θ=0.5 incl. rebuild",
+                   "synthetic two-disk encounter (seeded generator)",
+                   energy_theta=None, cpu=not args.no_cpu_baseline)
 
-    sim = engine.Simulation(pos, vel, dt=dt, eps=eps, theta=args.theta, force=args.force,
+
+def bh_main(args, pos, vel, eps, dt, what, metric, data, energy_theta=None, cpu=True):
+    """Barnes-Hut leapfrog steps of one system: timed steps, phase times,
+    walk roofline from the per-target walk counts, accuracy vs FP64 direct
+    summation, optional tree-energy drift over the timed steps and the CPU
+    baseline."""
+    from nbodyssey_amd import engine
+    n = len(pos)
+    sim = engine.Simulation(pos, vel, dt=dt, eps=eps, theta=args.theta, force="bh",
                             traverse=args.traverse, group_width=args.group_width,
                             key_bits=args.key_bits)
-    if args.force == "naive":
-        return naive_main(args, sim, eps)
     if args.graph:
         sim.set_graph(True)
     sim.compute_forces()              # a(x0)
     sim.step(args.warmup)
     sim.synchronize()
     print("bench: warmup done", file=sys.stderr, flush=True)
+    energy = None
+    if energy_theta:
+        e0 = sim.total_energy_tree(energy_theta)
 
     sim.reset_force_timing()
     sim.synchronize()
@@ -346,9 +382,14 @@ def main():
     wall = time.perf_counter() - t0
     phase, calls = sim.bh_phase_ms()
     ms_per_step = ms_events / args.steps
-    value = args.n * args.steps / (ms_events * 1e-3)
+    value = n * args.steps / (ms_events * 1e-3)
     print(f"bench: {args.steps} steps {ms_events:.2f} ms (wall {wall * 1e3:.2f} ms)",
           file=sys.stderr, flush=True)
+    if energy_theta:
+        e1 = sim.total_energy_tree(energy_theta)
+        energy = {"method": f"tree potential at theta={energy_theta} (nb_total_energy_tree)",
+                  "E_start": float(e0[0]), "E_end": float(e1[0]),
+                  "rel_drift": float((e1[0] - e0[0]) / abs(e0[0])), "steps": args.steps}
 
     # algorithmic flops of the walk on the final tree: per-target walk counts
     inter, opens = sim.traverse_counts("thread")
@@ -360,21 +401,18 @@ def main():
     lane_inter = ginter.astype(np.float64).sum() * width if width > 1 else ginter.sum()
 
     # accuracy of the final state's forces vs FP64 direct summation on the GPU
-    idx = np.random.default_rng(0).choice(args.n, args.accuracy_targets,
+    idx = np.random.default_rng(0).choice(n, args.accuracy_targets,
                                           replace=False).astype(np.int32)
     err = sim.force_error(idx)
     accuracy = {"median_rel_force_err_vs_fp64": float(np.median(err)),
                 "p99_rel_force_err_vs_fp64": float(np.quantile(err, 0.99)),
                 "targets": int(args.accuracy_targets),
                 "reference": "FP64 direct summation on the GPU (nb_direct_f64)"}
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline(args.n, args.cpu_sample, eps, dt)
+    cpu_line = cpu_baseline(n, args.cpu_sample, eps, dt) if cpu else None
 
     names = ["bbox", "morton", "sort", "build", "com", "traverse"]
     out = {
-        "metric": "particle-steps/s @1M BH θ=0.5 incl. rebuild",
+        "metric": metric,
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -385,11 +423,11 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic two-disk encounter (seeded generator)",
-        "config": {"workload": f"two-disk {args.n} particles, Barnes-Hut LBVH rebuilt every step, "
+        "data": data,
+        "config": {"workload": f"{what}, Barnes-Hut LBVH rebuilt every step, "
                                f"{args.key_bits}-bit keys, theta={args.theta}, eps={eps}, dt={dt}, "
                                f"walk={args.traverse}{'' if width == 1 else width}",
-                   "n": args.n, "theta": args.theta, "key_bits": args.key_bits,
+                   "n": n, "theta": args.theta, "key_bits": args.key_bits,
                    "traverse": args.traverse, "group_width": width},
         "phase_ms": {k: phase[i] / max(calls, 1) for i, k in enumerate(names)},
         "roofline": {"bound": "valu", "kernel": "walk",
@@ -399,10 +437,12 @@ def main():
                      "flops_per_launch": flops, "walk_ms": walk_ms,
                      "per_target_interactions": float(inter.mean()),
                      "per_target_opens": float(opens.mean()),
-                     "group_lane_interactions_per_target": float(lane_inter / args.n)},
+                     "group_lane_interactions_per_target": float(lane_inter / n)},
         "accuracy": accuracy,
-        "cpu_baseline": cpu,
+        "cpu_baseline": cpu_line,
     }
+    if energy is not None:
+        out["energy"] = energy
     print(json.dumps(out))
     sim.close()
 

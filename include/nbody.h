@@ -157,6 +157,16 @@ int nb_sort_pairs(const uint64_t* keys, int n, int key_bits, uint64_t* keys_sort
 int nb_karras_build(const uint64_t* keys_sorted, int n, int key_bits, int32_t* children,
                     int32_t* parent);
 
+/* total_energy (src/integrator.cu:43-91) at O(n log n) cost, the policy for
+ * large n where the exact O(n^2) nb_total_energy takes minutes: KE exactly,
+ * PE = -G/2 sum_i m_i phi_i with phi_i from a wave64 tree walk at `theta`
+ * (in (0, 0.577), so every cell containing the target is opened and the self
+ * term is removed exactly as the reference does, m_i / eps).  FP32 per-pair
+ * potential, FP64 sums.  Needs the Barnes-Hut force module, epsilon > 0, a
+ * single-system engine; builds the tree first if the last force evaluation
+ * did not. */
+int nb_total_energy_tree(nb_engine* h, float theta, double out[3]);
+
 /* FP64 direct-summation accelerations of nt targets (original particle
  * indices) from the engine's current positions: 3 doubles per target,
  * G * sum_{j != i} m_j (r_j - r_i) / (r^2 + eps^2)^{3/2} in FP64 on the

@@ -91,6 +91,7 @@ SIGNATURES = {
     "nb_ensemble_roots": (_i, [_vp, _vp]),
     "nb_system_energy": (_i, [_vp, _vp]),
     "nb_direct_f64": (_i, [_vp, _vp, _i, _vp]),
+    "nb_total_energy_tree": (_i, [_vp, ctypes.c_float, _vp]),
 }
 
 _lib = None

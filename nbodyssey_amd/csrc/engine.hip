@@ -404,6 +404,34 @@ class Engine {
     NB_HIP(hipStreamSynchronize(stream));
   }
 
+  // Energy with the potential from a tree walk at theta_e (the O(n) cost
+  // policy for large n): builds the tree for the current positions if the
+  // last force pass did not, then walks it once more for the potential.
+  void tree_energy(float theta_e, double out[3]) {
+    if (nsys > 0) throw Error(kErrInvalid, "tree energy is for single systems (use the exact "
+                                           "per-system energy of an ensemble)");
+    if (p.force != 1) throw Error(kErrInvalid, "tree energy needs the Barnes-Hut force module");
+    if (!(theta_e > 0.f && theta_e * theta_e < 1.f / 3.f))
+      throw Error(kErrInvalid, "tree-energy theta must be in (0, 0.577)");
+    if (!(p.eps > 0.f)) throw Error(kErrInvalid, "tree energy needs epsilon > 0");
+    if (n < 2) throw Error(kErrInvalid, "tree energy needs at least 2 particles");
+    if (!tree_valid) {
+      const bool saved = timing_on;
+      timing_on = false;
+      force_pass(false);
+      timing_on = saved;
+    }
+    const int parts = tree_energy_partials(n);
+    if (phi.count < (size_t)n) phi.alloc((size_t)n);
+    if ((int)en_partial.count < 2 * parts) en_partial.alloc((size_t)2 * parts);
+    if (en_out.count < 3) en_out.alloc(3);
+    WalkArgs w = walk_args();
+    launch_tree_energy(stream, w, theta_e * theta_e, vel, phi.p, en_partial.p, en_out.p);
+    NB_HIP(hipMemcpyAsync(out, en_out.p, 3 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    NB_HIP(hipStreamSynchronize(stream));
+    check_overflow();
+  }
+
   // FP64 direct-summation accelerations of the given targets (original
   // indices); an ensemble target's sources are its own system.
   void direct_f64(const int32_t* targets, int nt, double* acc3) {
@@ -567,6 +595,7 @@ class Engine {
   long bh_calls = 0;
 
   DevBuf<float4> own_pos, own_vel, own_acc;
+  DevBuf<float> phi;
   DevBuf<float4> ap_partial;
   DevBuf<double> en_partial, en_out;
   int bh_cap = 0, max_tiles = 0;
@@ -731,6 +760,14 @@ int nb_ensemble_roots(nb_engine* h, int32_t* roots) {
     if (!e.tree_valid) throw nb::Error(nb::kErrState, "no Barnes-Hut tree has been built");
     e.sync();
     NB_HIP(hipMemcpy(roots, e.roots.p, e.nsys * sizeof(int), hipMemcpyDeviceToHost));
+  });
+}
+
+int nb_total_energy_tree(nb_engine* h, float theta, double out[3]) {
+  return guarded([&] {
+    need(h, "engine");
+    need(out, "out");
+    h->e->tree_energy(theta, out);
   });
 }
 

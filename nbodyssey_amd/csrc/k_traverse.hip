@@ -139,14 +139,32 @@ constexpr int kWaveList = 128;     // LDS interaction-list entries per wave
 constexpr float kMacMargin = 1e-5f;
 constexpr int kReps = 8;           // representative targets for the open shortcut
 
-template <bool COMP>
+// POT: also accumulate the potential sum of m / sqrt(r^2 + eps^2) into pot
+// (m * inv is the first product of the force factor, so the force bits are
+// unchanged).
+template <bool COMP, bool POT>
 __device__ __forceinline__ void flush_list(const float4* list, int cnt, float bx, float by,
                                            float bz, float eps2, float& ax, float& ay, float& az,
-                                           float& cx, float& cy, float& cz) {
+                                           float& cx, float& cy, float& cz, float& pot) {
   float px = 0.f, py = 0.f, pz = 0.f;
   if (!COMP) { px = ax; py = ay; pz = az; }
+  if (POT) {
 #pragma unroll 8
-  for (int k = 0; k < cnt; ++k) accumulate_src(list[k], bx, by, bz, eps2, px, py, pz);
+    for (int k = 0; k < cnt; ++k) {
+      const float4 src = list[k];
+      const float dx = src.x - bx, dy = src.y - by, dz = src.z - bz;
+      const float inv = rsq(fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2))));
+      const float mi = src.w * inv;
+      const float f = mi * inv * inv;
+      px = fmaf(dx, f, px);
+      py = fmaf(dy, f, py);
+      pz = fmaf(dz, f, pz);
+      pot += mi;
+    }
+  } else {
+#pragma unroll 8
+    for (int k = 0; k < cnt; ++k) accumulate_src(list[k], bx, by, bz, eps2, px, py, pz);
+  }
   if (COMP) {
     kahan_add(ax, cx, px);
     kahan_add(ay, cy, py);
@@ -160,13 +178,13 @@ __device__ __forceinline__ void flush_list(const float4* list, int cnt, float bx
 // walking the subtree under `root`, on one wave; `slot` indexes the count
 // outputs.  Dead lanes (lane >= count) repeat the group's last target, so the
 // group box and the representatives are those of the live targets.
-template <bool COUNT, bool COMP>
+template <bool COUNT, bool COMP, bool POT = false>
 __device__ __forceinline__ void walk_group64(
     int t0, int count, int root, int slot, const float4* __restrict__ pos_sorted,
     const unsigned* __restrict__ perm, int n, const Node* __restrict__ nodes, float theta2,
     float eps2, float G, float4* __restrict__ acc, float4* __restrict__ vel, float half_dt,
     int* __restrict__ counts, int* __restrict__ opens, int* overflow, int* error, int* stk,
-    float4* list) {
+    float4* list, float* __restrict__ phi = nullptr) {
   const int lane = threadIdx.x & 63;
   const bool live = lane < count;
   const int t = t0 + (live ? lane : count - 1);
@@ -192,7 +210,7 @@ __device__ __forceinline__ void walk_group64(
     rz[r] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b.z), src_lane));
   }
 
-  float ax = 0.f, ay = 0.f, az = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
+  float ax = 0.f, ay = 0.f, az = 0.f, cx = 0.f, cy = 0.f, cz = 0.f, pot = 0.f;
   int sp = 1, nlist = 0;
   long ni = 0, no = 0;
   const unsigned nodes_total = 2u * (unsigned)n - 1u;
@@ -290,17 +308,105 @@ __device__ __forceinline__ void walk_group64(
     if ((src_mask >> lane) & 1ull) list[nlist + __popcll(src_mask & lt)] = src;
     nlist += __popcll(src_mask);
     if (nlist >= 64) {
-      flush_list<COMP>(list, 64, b.x, b.y, b.z, eps2, ax, ay, az, cx, cy, cz);
+      flush_list<COMP, POT>(list, 64, b.x, b.y, b.z, eps2, ax, ay, az, cx, cy, cz, pot);
       nlist -= 64;
       if (lane < nlist) list[lane] = list[64 + lane];
     }
   }
-  if (nlist > 0) flush_list<COMP>(list, nlist, b.x, b.y, b.z, eps2, ax, ay, az, cx, cy, cz);
+  if (nlist > 0)
+    flush_list<COMP, POT>(list, nlist, b.x, b.y, b.z, eps2, ax, ay, az, cx, cy, cz, pot);
   if (COUNT && lane == 0) {
     counts[slot] = (int)ni;
     opens[slot] = (int)no;
   }
+  if (POT && live) phi[t] = pot;
   if (live) finish(t, n, ax, ay, az, G, perm, acc, vel, half_dt);
+}
+
+// Potential walk (tree energy): the wave64 walk at its own theta writing, per
+// sorted target t, phi[t] = sum over its interaction list of m / sqrt(r^2 +
+// eps^2) -- the target's own leaf included once (for theta^2 < 1/3 every node
+// containing the target is opened).
+__global__ __launch_bounds__(kBlock) void walk_potential_kernel(
+    const float4* __restrict__ pos_sorted, const unsigned* __restrict__ perm, int n,
+    const Node* __restrict__ nodes, float theta2, float eps2, float* __restrict__ phi,
+    int* overflow, int* error) {
+  constexpr int WAVES = kBlock / 64;
+  __shared__ int s_stack[WAVES][kWaveStack];
+  __shared__ float4 s_list[WAVES][kWaveList];
+  const int w = threadIdx.x >> 6;
+  const int gbase = xcd_block() * blockDim.x + (threadIdx.x & ~63);
+  if (gbase >= n) return;  // wave-uniform
+  walk_group64<false, false, true>(gbase, min(64, n - gbase), 0, gbase >> 6, pos_sorted, perm, n,
+                                   nodes, theta2, eps2, 1.f, nullptr, nullptr, 0.f, nullptr,
+                                   nullptr, overflow, error, s_stack[w], s_list[w], phi);
+}
+
+// Energy from the potential walk, FP64 per block: KE = sum m v^2 / 2,
+// PE = -G/2 sum m (phi - m / eps), the self term removed with the same FP32
+// value the walk added (m * rsq(eps^2)).
+__global__ __launch_bounds__(kBlock) void tree_energy_kernel(
+    const float4* __restrict__ pos_sorted, const unsigned* __restrict__ perm,
+    const float4* __restrict__ vel, const float* __restrict__ phi, int n, float eps2, float G,
+    double* __restrict__ partial) {
+  __shared__ double s_red[2][kBlock / 64];
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  double ke = 0.0, pe = 0.0;
+  if (t < n) {
+    const float4 b = pos_sorted[t];
+    const unsigned o = perm[t];
+    const float4 v = vel[o < (unsigned)n ? o : 0];
+    const double m = (double)b.w;
+    const double vx = v.x, vy = v.y, vz = v.z;
+    ke = 0.5 * m * (vx * vx + vy * vy + vz * vz);
+    const float self = b.w * rsq(eps2);
+    pe = -0.5 * (double)G * m * ((double)phi[t] - (double)self);
+  }
+  ke = wave_sum(ke);
+  pe = wave_sum(pe);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    s_red[0][w] = ke;
+    s_red[1][w] = pe;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, c = 0.0;
+    for (int k = 0; k < kBlock / 64; ++k) {
+      a += s_red[0][k];
+      c += s_red[1][k];
+    }
+    partial[2 * blockIdx.x] = a;
+    partial[2 * blockIdx.x + 1] = c;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void tree_energy_final_kernel(
+    const double* __restrict__ partial, int count, double* out) {
+  __shared__ double s_red[2][kBlock / 64];
+  double ke = 0.0, pe = 0.0;
+  for (int k = threadIdx.x; k < count; k += kBlock) {
+    ke += partial[2 * k];
+    pe += partial[2 * k + 1];
+  }
+  ke = wave_sum(ke);
+  pe = wave_sum(pe);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    s_red[0][w] = ke;
+    s_red[1][w] = pe;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, c = 0.0;
+    for (int k = 0; k < kBlock / 64; ++k) {
+      a += s_red[0][k];
+      c += s_red[1][k];
+    }
+    out[0] = a + c;
+    out[1] = a;
+    out[2] = c;
+  }
 }
 
 template <bool COUNT, bool COMP>
@@ -472,6 +578,22 @@ void launch_walk_items(hipStream_t s, const WalkArgs& a, const int4* items, int 
   else
     walk_items_kernel<false, false><<<grid, kBlock, 0, s>>>(NB_ITEM_ARGS);
 #undef NB_ITEM_ARGS
+  NB_LAUNCH_CHECK();
+}
+
+int tree_energy_partials(int n) { return (n + kBlock - 1) / kBlock; }
+
+void launch_tree_energy(hipStream_t s, const WalkArgs& a, float theta2, const float4* vel,
+                        float* phi, double* partial, double* out) {
+  if (a.n < 2) throw Error(kErrInvalid, "tree energy needs at least 2 particles");
+  const int g = (a.n + kBlock - 1) / kBlock;
+  walk_potential_kernel<<<g, kBlock, 0, s>>>(a.pos_sorted, a.perm, a.n, a.nodes, theta2, a.eps2,
+                                             phi, a.overflow, a.error);
+  NB_LAUNCH_CHECK();
+  tree_energy_kernel<<<g, kBlock, 0, s>>>(a.pos_sorted, a.perm, vel, phi, a.n, a.eps2, a.G,
+                                          partial);
+  NB_LAUNCH_CHECK();
+  tree_energy_final_kernel<<<1, kBlock, 0, s>>>(partial, g, out);
   NB_LAUNCH_CHECK();
 }
 

@@ -162,6 +162,12 @@ struct WalkArgs {
   bool compensated;   // Kahan-add each block of interactions (wave64 walk)
 };
 void launch_walk(hipStream_t s, int traverse, int group_width, const WalkArgs& a);
+// Energy from a potential walk at theta2 (< 1/3) on the built tree:
+// phi (float x n, sorted order) and partial (2 x tree_energy_partials(n)
+// doubles) are scratch; out = (total, kinetic, potential), FP64.
+int tree_energy_partials(int n);
+void launch_tree_energy(hipStream_t s, const WalkArgs& a, float theta2, const float4* vel,
+                        float* phi, double* partial, double* out);
 // Persistent wave64 walk over ensemble work items (system, first target,
 // count <= 64, 0), each system from its own subtree root; `queue` must be 0.
 void launch_walk_items(hipStream_t s, const WalkArgs& a, const int4* items, int nitems,

@@ -119,6 +119,13 @@ class Simulation:
         check(self._L.nb_total_energy(self._h, ptr(out)))
         return out
 
+    def total_energy_tree(self, theta=0.3):
+        """(total, kinetic, potential) with the potential from a tree walk at
+        theta (O(n log n); Barnes-Hut engines only)."""
+        out = np.zeros(3)
+        check(self._L.nb_total_energy_tree(self._h, float(theta), ptr(out)))
+        return out
+
     def direct_f64(self, targets):
         """FP64 direct-sum accelerations (nt, 3) of the given original indices."""
         t = np.ascontiguousarray(targets, dtype=np.int32)

@@ -184,6 +184,125 @@ def ensemble_encounters(nsys=256, seed=0, n_min=2048, n_max=65536, eps=0.05):
     return systems, meta
 
 
+def _jeans_sigma2(r_grid, rho, m_total_enc):
+    """Isotropic Jeans equation, sigma_r^2(r) = (1/rho) int_r^inf rho G M(<r') / r'^2 dr'
+    (G = 1), by the trapezoid rule on the grid from the outside in."""
+    integrand = rho * m_total_enc / (r_grid * r_grid)
+    seg = 0.5 * (integrand[1:] + integrand[:-1]) * np.diff(r_grid)
+    tail = np.concatenate([np.cumsum(seg[::-1])[::-1], [0.0]])
+    return tail / np.maximum(rho, 1e-300)
+
+
+def composite_galaxy(n, rng, m_total=1.0, f_disk=0.6, f_bulge=0.1, f_halo=0.3, rd=1.0,
+                     z0=0.1, a_bulge=0.2, a_halo=5.0, eps=0.02, m_cut=0.99):
+    """Disk + Hernquist bulge + Plummer halo in equilibrium (own frame, float64).
+
+    * disk: exponential surface density (R ~ Gamma(2, rd), resampled above
+      6 rd), z ~ N(0, z0), circular speed sqrt(M(<R) / sqrt(R^2 + eps^2))
+      from the spherical enclosed mass of all three components, plus
+      N(0, 0.05 v_c) per component;
+    * bulge: Hernquist, M(<r) = M r^2 / (r + a)^2, r by inverse CDF;
+    * halo: Plummer, M(<r) = M r^3 / (r^2 + a^2)^(3/2), r by inverse CDF;
+      both radially truncated at enclosed-mass fraction m_cut, isotropic
+      directions, velocities Gaussian with the isotropic Jeans dispersion
+      of each component in the total potential (disk mass taken as
+      spherically distributed, M_d(<r) = M_d [1 - (1 + r/rd) e^(-r/rd)]).
+    Particle counts are split by mass fraction (equal particle masses).
+    Returns (pos (n,3), vel (n,3), mass (n,)).
+    """
+    nb_ = int(round(n * f_bulge))
+    nh = int(round(n * f_halo))
+    nd = n - nb_ - nh
+    mp = m_total / n
+    md, mb, mh = m_total * f_disk, m_total * f_bulge, m_total * f_halo
+
+    def m_disk(r):
+        x = r / rd
+        return md * (1.0 - (1.0 + x) * np.exp(-x))
+
+    def m_bulge(r):
+        return mb * r * r / (r + a_bulge) ** 2
+
+    def m_halo(r):
+        return mh * r ** 3 / (r * r + a_halo * a_halo) ** 1.5
+
+    def m_enc(r):
+        return m_disk(r) + m_bulge(r) + m_halo(r)
+
+    # disk
+    R = rng.gamma(2.0, rd, size=nd)
+    bad = R > 6.0 * rd
+    while np.any(bad):
+        R[bad] = rng.gamma(2.0, rd, size=int(bad.sum()))
+        bad = R > 6.0 * rd
+    phi = rng.uniform(0.0, 2.0 * np.pi, size=nd)
+    z = rng.normal(0.0, z0, size=nd)
+    vc = np.sqrt(m_enc(R) * R * R / (R * R + eps * eps) ** 1.5)
+    pd = np.stack([R * np.cos(phi), R * np.sin(phi), z], axis=1)
+    vd = np.stack([-vc * np.sin(phi), vc * np.cos(phi), np.zeros(nd)], axis=1)
+    vd += rng.normal(0.0, 1.0, size=(nd, 3)) * (0.05 * vc)[:, None]
+
+    def iso_dirs(k):
+        u = rng.uniform(-1.0, 1.0, size=k)
+        t = rng.uniform(0.0, 2.0 * np.pi, size=k)
+        s = np.sqrt(1.0 - u * u)
+        return np.stack([s * np.cos(t), s * np.sin(t), u], axis=1)
+
+    # spheroids: inverse-CDF radii, truncated at m_cut of their mass
+    q = rng.uniform(0.0, m_cut, size=nb_)
+    sq = np.sqrt(q)
+    rb = a_bulge * sq / (1.0 - sq)
+    q = rng.uniform(0.0, m_cut, size=nh)
+    rh = a_halo / np.sqrt(q ** (-2.0 / 3.0) - 1.0)
+    r_grid = np.logspace(-4, np.log10(max(rh.max(), rb.max(), 10.0) * 4.0), 4000)
+    menc = m_enc(r_grid)
+    rho_b = mb * a_bulge / (2 * np.pi * r_grid * (r_grid + a_bulge) ** 3)
+    rho_h = 3 * mh / (4 * np.pi * a_halo ** 3) * (1 + r_grid ** 2 / a_halo ** 2) ** -2.5
+    sig_b = np.sqrt(np.maximum(_jeans_sigma2(r_grid, rho_b, menc), 0.0))
+    sig_h = np.sqrt(np.maximum(_jeans_sigma2(r_grid, rho_h, menc), 0.0))
+    pb = iso_dirs(nb_) * rb[:, None]
+    vb = rng.normal(0.0, 1.0, size=(nb_, 3)) * np.interp(rb, r_grid, sig_b)[:, None]
+    ph = iso_dirs(nh) * rh[:, None]
+    vh = rng.normal(0.0, 1.0, size=(nh, 3)) * np.interp(rh, r_grid, sig_h)[:, None]
+    pos = np.concatenate([pd, pb, ph])
+    vel = np.concatenate([vd, vb, vh])
+    return pos, vel, np.full(n, mp)
+
+
+def composite_encounter(n_total=1 << 25, seed=5, pericentre=3.0, separation=30.0,
+                        inclination_deg=45.0, eps=0.02):
+    """Two equal composite galaxies (``composite_galaxy``, mass 1 each) on a
+    parabolic orbit of the given pericentre, starting ``separation`` apart;
+    the second is tilted by the inclination about x and counter-rotates."""
+    rng = np.random.default_rng(seed)
+    na = n_total - n_total // 2
+    nb_ = n_total // 2
+    pa, va, ma = composite_galaxy(na, rng, eps=eps)
+    pb, vb, mb = composite_galaxy(nb_, rng, eps=eps)
+    ma[:] = mb[:] = 2.0 / n_total
+    vb = -vb   # counter-rotating (the spheroids' isotropic velocities stay isotropic)
+    incl = math.radians(inclination_deg)
+    pb, vb = _rotate_x(pb, incl), _rotate_x(vb, incl)
+    m_tot, d, rp = 2.0, float(separation), float(pericentre)
+    v2 = 2.0 * m_tot / d
+    vt = math.sqrt(2.0 * m_tot * rp) / d
+    vr = -math.sqrt(max(v2 - vt * vt, 0.0))
+    r_rel = np.array([d, 0.0, 0.0])
+    v_rel = np.array([vr, vt, 0.0])
+    pa += -0.5 * r_rel
+    va += -0.5 * v_rel
+    pb += 0.5 * r_rel
+    vb += 0.5 * v_rel
+    pos4 = np.empty((n_total, 4), np.float32)
+    vel4 = np.zeros((n_total, 4), np.float32)
+    pos4[:na, :3] = pa
+    pos4[na:, :3] = pb
+    pos4[:, 3] = 2.0 / n_total
+    vel4[:na, :3] = va
+    vel4[na:, :3] = vb
+    return pos4, vel4
+
+
 # Workloads the benchmark and the tests name (sizes, softening, step, theta).
 WORKLOADS = {
     "two-disk-12k-naive": dict(n=12000, seed=0, eps=0.05, dt=0.01, steps=200, theta=0.5,
@@ -194,6 +313,8 @@ WORKLOADS = {
                                   theta=0.5, force="bh", key_bits=30),
     "ensemble-256-parabolic": dict(nsys=256, seed=0, eps=0.05, dt=0.01, steps=200, theta=0.5,
                                    force="bh"),
+    "composite-33M-63bit": dict(n=1 << 25, seed=5, eps=0.02, dt=0.002, steps=50, theta=0.6,
+                                force="bh", key_bits=63),
 }
 
 
