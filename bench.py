@@ -113,8 +113,8 @@ def ensemble_main(args):
     """Batched ensemble of independent parabolic two-disk encounters against
     the same systems run one at a time on the plain path.
 
-    Batched: one engine (nb_create_ensemble), every step captured as a HIP
-    graph.  Serial: one plain engine per system (Barnes-Hut, wave64 walk,
+    Batched: one engine (nb_create_ensemble), launch configuration chosen by
+    the autotuner (persistent-walk workgroups, graph replay).  Serial: one plain engine per system (Barnes-Hut, wave64 walk,
     30-bit keys, graph mode: the fastest plain configuration), started from
     the batched run's final state, timed per system and summed.  Both compute
     bitwise the same forces for a given state (the forest holds each system's
@@ -133,6 +133,12 @@ def ensemble_main(args):
     ens = engine.Ensemble(systems, dt=dt, eps=eps, theta=theta)
     ens.set_graph(not args.no_graph)
     ens.compute_forces()
+    tuned = None
+    if not args.no_graph:
+        # autotuner: persistent-walk workgroups x graph replay, within 1% median error
+        choice, log = ens.autotune(steps=3, max_median_err=0.01)
+        tuned = {"choice": choice, "log": log}
+        print("bench: autotune " + log[0], file=sys.stderr, flush=True)
     ens.step(args.warmup)
     ens.synchronize()
     ms_batched = ens.time_steps(args.steps)
@@ -201,6 +207,7 @@ def ensemble_main(args):
                    "how": "each system alone on the plain path (bh, wave64, 30-bit keys, "
                           "graph mode), summed"},
         "batched_over_serial": value / serial_value,
+        "autotune": tuned,
         "batched_forces_bitwise_equal_serial": not mismatched,
         "phase_ms": {k: phase[i] / max(calls, 1) for i, k in enumerate(names)},
         "roofline": {"bound": "valu", "kernel": "walk_items",

@@ -167,6 +167,29 @@ int nb_karras_build(const uint64_t* keys_sorted, int n, int key_bits, int32_t* c
  * did not. */
 int nb_total_energy_tree(nb_engine* h, float theta, double out[3]);
 
+/* Autotuner (no counterpart in the reference): times candidate launch
+ * configurations for `steps` leapfrog steps each on the current state, and
+ * keeps the fastest whose median force error against FP64 direct summation
+ * (256 fixed targets) is <= max_median_err (if none is, the most accurate).
+ * Candidates: Barnes-Hut -- walk group width 64/32/16 x key bits 30/63 x
+ * graph replay off/on; all-pairs -- source split x graph; ensemble --
+ * persistent-walk workgroups x graph.  The particle state is saved and
+ * restored (tuning does not advance the simulation); the choice stays set
+ * on the engine.  log (may be NULL) receives one line per candidate, the
+ * chosen one first. */
+typedef struct nb_tune_result {
+  int group_width;
+  int key_bits;
+  int graph;           /* 1: steps replay a captured HIP graph */
+  int allpairs_split;  /* 0: automatic */
+  int walk_grid;       /* ensemble persistent-walk workgroups, 0: automatic */
+  double ms_per_step;  /* of the chosen configuration */
+  double median_err;   /* of the chosen configuration */
+  int candidates;
+} nb_tune_result;
+int nb_autotune(nb_engine* h, int steps, float max_median_err, nb_tune_result* out, char* log,
+                int log_len);
+
 /* FP64 direct-summation accelerations of nt targets (original particle
  * indices) from the engine's current positions: 3 doubles per target,
  * G * sum_{j != i} m_j (r_j - r_i) / (r^2 + eps^2)^{3/2} in FP64 on the

@@ -40,6 +40,19 @@ class NbParams(ctypes.Structure):
     ]
 
 
+class NbTuneResult(ctypes.Structure):
+    _fields_ = [
+        ("group_width", ctypes.c_int),
+        ("key_bits", ctypes.c_int),
+        ("graph", ctypes.c_int),
+        ("allpairs_split", ctypes.c_int),
+        ("walk_grid", ctypes.c_int),
+        ("ms_per_step", ctypes.c_double),
+        ("median_err", ctypes.c_double),
+        ("candidates", ctypes.c_int),
+    ]
+
+
 class NbError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"libnbody error {code}: {msg}")
@@ -92,6 +105,7 @@ SIGNATURES = {
     "nb_system_energy": (_i, [_vp, _vp]),
     "nb_direct_f64": (_i, [_vp, _vp, _i, _vp]),
     "nb_total_energy_tree": (_i, [_vp, ctypes.c_float, _vp]),
+    "nb_autotune": (_i, [_vp, _i, ctypes.c_float, ctypes.POINTER(NbTuneResult), _cp, _i]),
 }
 
 _lib = None

@@ -15,6 +15,7 @@
 #include "nbody/simulation.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -187,7 +188,7 @@ class Engine {
       for (int k = 1; k <= kPhases; ++k) mark(ce, k);
     } else {
       if (step) launch_kick_drift(stream, pos, vel, acc, n, half, p.dt, false, nullptr);
-      int split = allpairs_split(n);
+      const int split = ap_split > 0 ? ap_split : allpairs_split(n);
       launch_allpairs(stream, pos, n, eps2, p.G, acc, ap_partial.p, split, p.precision == 1,
                       step ? vel : nullptr, half);
       mark(ce, 1);
@@ -219,7 +220,7 @@ class Engine {
     WalkArgs w = walk_args();
     w.vel = step ? vel : nullptr;
     w.half_dt = half;
-    launch_walk_items(stream, w, items.p, nitems, roots.p, &state.p->queue, 0);
+    launch_walk_items(stream, w, items.p, nitems, roots.p, &state.p->queue, walk_grid);
     mark(ce, kPhWalk + 1);
     tree_valid = true;
   }
@@ -313,7 +314,7 @@ class Engine {
   void prepare() {
     if ((p.force == 1 && n >= 2) || nsys > 0) ensure_bh();
     if (p.force == 0) {
-      const int split = allpairs_split(n);
+      const int split = ap_split > 0 ? ap_split : allpairs_split(n);
       if (split > 1 && ap_partial.count < (size_t)split * n) ap_partial.alloc((size_t)split * n);
     }
   }
@@ -402,6 +403,150 @@ class Engine {
     launch_energy(stream, pos, vel, n, p.eps * p.eps, p.G, en_partial.p, np, en_out.p);
     NB_HIP(hipMemcpyAsync(out, en_out.p, 3 * sizeof(double), hipMemcpyDeviceToHost, stream));
     NB_HIP(hipStreamSynchronize(stream));
+  }
+
+  // ---------------------------------------------------------------- autotuner
+  // Times candidate launch configurations on the current state and keeps the
+  // fastest whose median force error against FP64 direct summation (256
+  // fixed targets) stays within max_err; the particle state is saved and
+  // restored, so tuning does not advance the simulation.
+  //   Barnes-Hut: walk group width {64, 32, 16} x key bits {30, 63} x graph
+  //               replay {off, on};
+  //   all-pairs:  source split {auto, 2x, 1/2, 1} x graph {off, on};
+  //   ensemble:   persistent-walk grid {resident, 3/4, 1/2} x graph {off, on}.
+  struct Candidate {
+    int group_width, key_bits, graph, split, grid;
+    double ms, err;
+  };
+  std::vector<Candidate> autotune(int steps, float max_err) {
+    if (steps < 1) throw Error(kErrInvalid, "autotune needs steps >= 1");
+    if (n < 2) throw Error(kErrInvalid, "autotune needs at least 2 particles");
+    sync();
+    const Params p0 = p;
+    const bool graph0 = graph_mode;
+    const int split0 = ap_split, grid0 = walk_grid;
+    DevBuf<float4> save;
+    save.alloc((size_t)3 * n);
+    NB_HIP(hipMemcpyAsync(save.p, pos, n * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+    NB_HIP(hipMemcpyAsync(save.p + n, vel, n * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+    NB_HIP(hipMemcpyAsync(save.p + 2 * (size_t)n, acc, n * sizeof(float4),
+                          hipMemcpyDeviceToDevice, stream));
+    const bool acc_valid0 = acc_valid;
+    auto restore = [&]() {
+      NB_HIP(hipMemcpyAsync(pos, save.p, n * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+      NB_HIP(hipMemcpyAsync(vel, save.p + n, n * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+      NB_HIP(hipMemcpyAsync(acc, save.p + 2 * (size_t)n, n * sizeof(float4),
+                            hipMemcpyDeviceToDevice, stream));
+      acc_valid = acc_valid0;
+      tree_valid = false;
+    };
+    // fixed error targets, spread over the particles
+    const int nt = std::min(n, 256);
+    std::vector<int32_t> tg(nt);
+    for (int k = 0; k < nt; ++k) tg[k] = (int32_t)(((long long)k * n) / nt);
+    std::vector<double> ref((size_t)3 * nt);
+    direct_f64(tg.data(), nt, ref.data());
+
+    std::vector<Candidate> cands;
+    if (nsys > 0) {
+      const int full = walk_items_resident();
+      for (int g : {full, full * 3 / 4, full / 2})
+        for (int gr = 0; gr < 2; ++gr) cands.push_back({64, 30, gr, 0, std::max(g, 1), 0, 0});
+    } else if (p.force == 1) {
+      for (int kb : {30, 63})
+        for (int w : {64, 32, 16})
+          for (int gr = 0; gr < 2; ++gr) cands.push_back({w, kb, gr, 0, 0, 0, 0});
+    } else {
+      const int a = allpairs_split(n);
+      std::vector<int> splits = {a, std::min(a * 2, 64), std::max(a / 2, 1), 1};
+      std::sort(splits.begin(), splits.end());
+      splits.erase(std::unique(splits.begin(), splits.end()), splits.end());
+      for (int sp : splits)
+        for (int gr = 0; gr < 2; ++gr) cands.push_back({p.group_width, p.key_bits, gr, sp, 0, 0, 0});
+    }
+    const bool timing0 = timing_on;
+    hipEvent_t e0, e1;
+    NB_HIP(hipEventCreate(&e0));
+    NB_HIP(hipEventCreate(&e1));
+    try {
+      for (Candidate& c : cands) {
+        Params q = p0;
+        if (nsys == 0 && q.force == 1) {
+          q.traverse = 1;
+          q.group_width = c.group_width;
+          q.key_bits = c.key_bits;
+        }
+        set_params(q);
+        ap_split = c.split;
+        walk_grid = c.grid;
+        drop_graph();
+        graph_mode = c.graph != 0;
+        timing_on = false;
+        restore();
+        // error of a(x) for this configuration
+        force_pass(false);
+        std::vector<float4> a(nt);
+        std::vector<double> err(nt);
+        for (int k = 0; k < nt; ++k)
+          NB_HIP(hipMemcpyAsync(&a[k], acc + tg[k], sizeof(float4), hipMemcpyDeviceToHost, stream));
+        NB_HIP(hipStreamSynchronize(stream));
+        for (int k = 0; k < nt; ++k) {
+          const double dx = a[k].x - ref[3 * k], dy = a[k].y - ref[3 * k + 1],
+                       dz = a[k].z - ref[3 * k + 2];
+          const double r = std::sqrt(ref[3 * k] * ref[3 * k] + ref[3 * k + 1] * ref[3 * k + 1] +
+                                     ref[3 * k + 2] * ref[3 * k + 2]);
+          err[k] = std::sqrt(dx * dx + dy * dy + dz * dz) / (r > 0 ? r : 1e-300);
+        }
+        std::nth_element(err.begin(), err.begin() + nt / 2, err.end());
+        c.err = err[nt / 2];
+        // time `steps` steps after one warmup step (captures the graph)
+        step(1);
+        NB_HIP(hipEventRecord(e0, stream));
+        step(steps);
+        NB_HIP(hipEventRecord(e1, stream));
+        NB_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        NB_HIP(hipEventElapsedTime(&ms, e0, e1));
+        c.ms = ms / steps;
+        check_overflow();
+      }
+    } catch (...) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      timing_on = timing0;
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    timing_on = timing0;
+    // fastest within the error bound; if none qualifies, the most accurate
+    int best = -1;
+    for (int k = 0; k < (int)cands.size(); ++k)
+      if (cands[k].err <= max_err && (best < 0 || cands[k].ms < cands[best].ms)) best = k;
+    if (best < 0) {
+      best = 0;
+      for (int k = 1; k < (int)cands.size(); ++k)
+        if (cands[k].err < cands[best].err) best = k;
+    }
+    const Candidate& c = cands[best];
+    Params q = p0;
+    if (nsys == 0 && q.force == 1) {
+      q.traverse = 1;
+      q.group_width = c.group_width;
+      q.key_bits = c.key_bits;
+    }
+    set_params(q);
+    ap_split = c.split;
+    walk_grid = c.grid;
+    drop_graph();
+    graph_mode = c.graph != 0;
+    restore();
+    NB_HIP(hipStreamSynchronize(stream));
+    (void)split0;
+    (void)grid0;
+    (void)graph0;
+    std::swap(cands[0], cands[best]);  // the chosen one first
+    return cands;
   }
 
   // Energy with the potential from a tree walk at theta_e (the O(n) cost
@@ -608,6 +753,8 @@ class Engine {
   DevBuf<QuantParams> quant;
   DevBuf<SortState> state;
   DevBuf<unsigned long long> lookback;
+  int ap_split = 0;    // all-pairs source split (0: automatic)
+  int walk_grid = 0;   // persistent-walk workgroups (0: all resident)
   // ensemble (nsys > 0)
   int nsys = 0, nitems = 0, ntiles_e = 0;
   std::vector<int> h_off;
@@ -760,6 +907,36 @@ int nb_ensemble_roots(nb_engine* h, int32_t* roots) {
     if (!e.tree_valid) throw nb::Error(nb::kErrState, "no Barnes-Hut tree has been built");
     e.sync();
     NB_HIP(hipMemcpy(roots, e.roots.p, e.nsys * sizeof(int), hipMemcpyDeviceToHost));
+  });
+}
+
+int nb_autotune(nb_engine* h, int steps, float max_median_err, nb_tune_result* out, char* log,
+                int log_len) {
+  return guarded([&] {
+    need(h, "engine");
+    std::vector<nb::Engine::Candidate> c = h->e->autotune(steps, max_median_err);
+    if (out) {
+      out->group_width = c[0].group_width;
+      out->key_bits = c[0].key_bits;
+      out->graph = c[0].graph;
+      out->allpairs_split = c[0].split;
+      out->walk_grid = c[0].grid;
+      out->ms_per_step = c[0].ms;
+      out->median_err = c[0].err;
+      out->candidates = (int)c.size();
+    }
+    if (log && log_len > 0) {
+      std::string t;
+      char line[160];
+      for (size_t k = 0; k < c.size(); ++k) {
+        std::snprintf(line, sizeof line,
+                      "%s width=%d keys=%d graph=%d split=%d grid=%d ms/step=%.4f median_err=%.3e\n",
+                      k == 0 ? "chosen " : "tried  ", c[k].group_width, c[k].key_bits, c[k].graph,
+                      c[k].split, c[k].grid, c[k].ms, c[k].err);
+        t += line;
+      }
+      std::snprintf(log, (size_t)log_len, "%s", t.c_str());
+    }
   });
 }
 

@@ -562,6 +562,8 @@ static int walk_items_grid() {
   return grid;
 }
 
+int walk_items_resident() { return walk_items_grid(); }
+
 void launch_walk_items(hipStream_t s, const WalkArgs& a, const int4* items, int nitems,
                        const int* roots, unsigned* queue, int grid) {
   if (nitems <= 0) return;

@@ -172,6 +172,9 @@ void launch_tree_energy(hipStream_t s, const WalkArgs& a, float theta2, const fl
 // count <= 64, 0), each system from its own subtree root; `queue` must be 0.
 void launch_walk_items(hipStream_t s, const WalkArgs& a, const int4* items, int nitems,
                        const int* roots, unsigned* queue, int grid);
+// Workgroups of the persistent walk that are resident at once (CUs x occupancy);
+// grid <= 0 above means this many.
+int walk_items_resident();
 
 // k_ensemble.hip: systems stored back to back, system s at [off[s], off[s+1]).
 // box: 8 u32 per system (orderable-float min xyz, max xyz, 2 unused), reset

@@ -119,6 +119,18 @@ class Simulation:
         check(self._L.nb_total_energy(self._h, ptr(out)))
         return out
 
+    def autotune(self, steps=5, max_median_err=0.01):
+        """Pick the fastest launch configuration within the error bound
+        (nb_autotune); returns (choice dict, per-candidate log lines)."""
+        r = _lib.NbTuneResult()
+        buf = ctypes.create_string_buffer(8192)
+        check(self._L.nb_autotune(self._h, int(steps), float(max_median_err), ctypes.byref(r),
+                                  buf, len(buf)))
+        choice = {k: getattr(r, k) for k, _ in _lib.NbTuneResult._fields_}
+        if self.params.get("force") in ("bh", _lib.FORCE_BH) and not hasattr(self, "nsys"):
+            self.params.update(traverse="warp", group_width=r.group_width, key_bits=r.key_bits)
+        return choice, buf.value.decode().splitlines()
+
     def total_energy_tree(self, theta=0.3):
         """(total, kinetic, potential) with the potential from a tree walk at
         theta (O(n log n); Barnes-Hut engines only)."""

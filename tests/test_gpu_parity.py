@@ -378,3 +378,43 @@ def test_tree_energy_errors():
         with pytest.raises(NbError):
             sim.total_energy_tree(0.6)
         assert np.isfinite(sim.total_energy_tree(0.3)).all()
+
+
+@pytest.mark.parametrize("force", ["bh", "naive"])
+def test_autotune_keeps_state_and_accuracy(force):
+    """nb_autotune restores the particle state (positions, velocities and the
+    next step's result are bitwise those of an untuned engine when the choice
+    equals the untuned configuration's physics: all-pairs splits only reorder
+    the FP32 sum, so positions are compared to 1e-6), and the chosen
+    configuration satisfies the error bound."""
+    pos, vel = ic.two_disk_encounter(20000, seed=41, eps=0.05)
+    kw = dict(dt=0.01, eps=0.05, theta=0.5, force=force, traverse="warp", group_width=64,
+              key_bits=30)
+    with engine.Simulation(pos, vel, **kw) as sim:
+        sim.compute_forces()
+        p0, v0 = sim.download()
+        choice, log = sim.autotune(steps=2, max_median_err=0.01)
+        print(choice, *log, sep="\n", flush=True)
+        p1, v1 = sim.download()
+        assert np.array_equal(p0, p1) and np.array_equal(v0, v1)
+        assert choice["median_err"] <= 0.01
+        assert len(log) == choice["candidates"] and log[0].startswith("chosen")
+        sim.step(2)
+        pt, _ = sim.download()
+    with engine.Simulation(pos, vel, **kw) as ref:
+        ref.compute_forces()
+        ref.step(2)
+        pr, _ = ref.download()
+    np.testing.assert_allclose(pt, pr, rtol=0, atol=1e-4)
+
+
+def test_autotune_ensemble():
+    systems, _ = ic.ensemble_encounters(12, seed=300, n_min=2048, n_max=8192)
+    with engine.Ensemble(systems, dt=0.01, eps=0.05, theta=0.5) as ens:
+        ens.compute_forces()
+        p0, _ = ens.download()
+        choice, log = ens.autotune(steps=2)
+        print(choice, *log, sep="\n", flush=True)
+        assert np.array_equal(ens.download()[0], p0)
+        assert choice["walk_grid"] > 0 and choice["median_err"] <= 0.01
+        ens.step(1)
