@@ -128,7 +128,7 @@ __global__ __launch_bounds__(kBlock) void walk_thread_kernel(
 //     every lane if size2 >= theta2 * dmax2 (farthest box corner), each with
 //     a 1e-5 relative margin so FP32 rounding cannot flip a lane's test;
 //   * otherwise the exact per-lane test in the canonical order, node by node
-//     with the node broadcast by readlane, reduced by ballot.
+//     with the node broadcast from LDS, reduced by ballot.
 // Opened nodes push their two children; accepted nodes and leaves append
 // their (x, y, z, m) to an LDS interaction list that all 64 lanes evaluate in
 // blocks of 64 (LDS broadcast reads, pure FP32 VALU), optionally Kahan-adding
@@ -267,24 +267,26 @@ __device__ __forceinline__ void walk_group64(
     }
     unsigned long long open_mask = __ballot(sure_open);
     unsigned long long amb = __ballot(ambiguous);
-    // exact vote for the nodes the prefilter could not settle, two at a time
-    while (amb) {
-      const int q0 = __builtin_ctzll(amb);
-      amb &= amb - 1ull;
-      const int q1 = amb ? __builtin_ctzll(amb) : q0;
-      amb &= amb - 1ull;
-      const float x0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(src.x), q0));
-      const float y0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(src.y), q0));
-      const float z0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(src.z), q0));
-      const float t0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(s2), q0));
-      const float x1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(src.x), q1));
-      const float y1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(src.y), q1));
-      const float z1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(src.z), q1));
-      const float t1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(s2), q1));
-      const bool op0 = live & !(t0 < theta2 * dist2(x0 - b.x, y0 - b.y, z0 - b.z));
-      const bool op1 = live & !(t1 < theta2 * dist2(x1 - b.x, y1 - b.y, z1 - b.z));
-      if (__ballot(op0)) open_mask |= 1ull << q0;
-      if (__ballot(op1)) open_mask |= 1ull << q1;
+    // exact vote for the nodes the prefilter could not settle: their (com,
+    // size2) are staged in the free upper half of the interaction list
+    // (fewer than 64 entries are pending between flushes) and read back by
+    // LDS broadcast, two nodes per iteration
+    if (amb) {
+      float4* vbuf = list + 64;
+      if (ambiguous) vbuf[__popcll(amb & lt)] = make_float4(src.x, src.y, src.z, s2);
+      const int na = __popcll(amb);
+      for (int k = 0; k < na; k += 2) {
+        const float4 v0 = vbuf[k];
+        const float4 v1 = vbuf[k + 1 < na ? k + 1 : k];
+        const int q0 = __builtin_ctzll(amb);
+        amb &= amb - 1ull;
+        const int q1 = amb ? __builtin_ctzll(amb) : q0;
+        amb &= amb - 1ull;
+        const bool op0 = live & !(v0.w < theta2 * dist2(v0.x - b.x, v0.y - b.y, v0.z - b.z));
+        const bool op1 = live & !(v1.w < theta2 * dist2(v1.x - b.x, v1.y - b.y, v1.z - b.z));
+        if (__ballot(op0)) open_mask |= 1ull << q0;
+        if (__ballot(op1)) open_mask |= 1ull << q1;
+      }
     }
     const unsigned long long node_mask = __ballot(is_node);
     const unsigned long long src_mask = (node_mask & ~open_mask) | __ballot(is_leaf);
