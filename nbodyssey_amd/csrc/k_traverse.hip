@@ -191,6 +191,7 @@ __device__ __forceinline__ void walk_group64(
   const int nm1 = n - 1;
   const float4 b = pos_sorted[t];
   const unsigned long long lt = (1ull << lane) - 1ull;
+  const unsigned long long live_mask = __builtin_amdgcn_ballot_w64(live);
 
   // bounding box of the live targets (dead lanes repeat the last target)
   float blx = wave_min(b.x), bly = wave_min(b.y), blz = wave_min(b.z);
@@ -265,31 +266,31 @@ __device__ __forceinline__ void walk_group64(
       sure_open = (s2 > acc_hi * dmax2) | !(s2 < theta2 * rmin);
       ambiguous = !sure_accept & !sure_open;
     }
-    unsigned long long open_mask = __ballot(sure_open);
-    unsigned long long amb = __ballot(ambiguous);
+    unsigned long long open_mask = __builtin_amdgcn_ballot_w64(sure_open);
+    unsigned long long amb = __builtin_amdgcn_ballot_w64(ambiguous);
     // exact vote for the nodes the prefilter could not settle: their (com,
     // size2) are staged in the free upper half of the interaction list
     // (fewer than 64 entries are pending between flushes) and read back by
     // LDS broadcast, two nodes per iteration
     if (amb) {
       float4* vbuf = list + 64;
-      if (ambiguous) vbuf[__popcll(amb & lt)] = make_float4(src.x, src.y, src.z, s2);
+      const int j = __popcll(amb & lt);  // this lane's rank among the ambiguous
+      if (ambiguous) vbuf[j] = make_float4(src.x, src.y, src.z, s2);
       const int na = __popcll(amb);
+      unsigned long long opened = 0;  // bit k: the k-th ambiguous node opens
       for (int k = 0; k < na; k += 2) {
         const float4 v0 = vbuf[k];
         const float4 v1 = vbuf[k + 1 < na ? k + 1 : k];
-        const int q0 = __builtin_ctzll(amb);
-        amb &= amb - 1ull;
-        const int q1 = amb ? __builtin_ctzll(amb) : q0;
-        amb &= amb - 1ull;
-        const bool op0 = live & !(v0.w < theta2 * dist2(v0.x - b.x, v0.y - b.y, v0.z - b.z));
-        const bool op1 = live & !(v1.w < theta2 * dist2(v1.x - b.x, v1.y - b.y, v1.z - b.z));
-        if (__ballot(op0)) open_mask |= 1ull << q0;
-        if (__ballot(op1)) open_mask |= 1ull << q1;
+        const bool op0 = !(v0.w < theta2 * dist2(v0.x - b.x, v0.y - b.y, v0.z - b.z));
+        const bool op1 = !(v1.w < theta2 * dist2(v1.x - b.x, v1.y - b.y, v1.z - b.z));
+        if (__builtin_amdgcn_ballot_w64(op0) & live_mask) opened |= 1ull << k;
+        if (__builtin_amdgcn_ballot_w64(op1) & live_mask) opened |= 2ull << k;
       }
+      open_mask |= __builtin_amdgcn_ballot_w64(ambiguous && ((opened >> j) & 1ull));
     }
-    const unsigned long long node_mask = __ballot(is_node);
-    const unsigned long long src_mask = (node_mask & ~open_mask) | __ballot(is_leaf);
+    const unsigned long long node_mask = __builtin_amdgcn_ballot_w64(is_node);
+    const unsigned long long src_mask =
+        (node_mask & ~open_mask) | __builtin_amdgcn_ballot_w64(is_leaf);
     if (COUNT) {
       ni += __popcll(src_mask);
       no += __popcll(node_mask);
