@@ -296,6 +296,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=16384)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--accuracy-targets", type=int, default=256)
+    ap.add_argument("--far16", action="store_true",
+                    help="FP16 far field for accepted cells (wave64 walk precision mode 2); "
+                         "reported with dtype f32+f16")
     ap.add_argument("--graph", action="store_true",
                     help="replay each step as a captured HIP graph (no per-phase timing)")
     ap.add_argument("--error-targets", type=int, default=65536)
@@ -370,7 +373,7 @@ def bh_main(args, pos, vel, eps, dt, what, metric, data, energy_theta=None, cpu=
     n = len(pos)
     sim = engine.Simulation(pos, vel, dt=dt, eps=eps, theta=args.theta, force="bh",
                             traverse=args.traverse, group_width=args.group_width,
-                            key_bits=args.key_bits)
+                            key_bits=args.key_bits, far16=args.far16)
     if args.graph:
         sim.set_graph(True)
     sim.compute_forces()              # a(x0)
@@ -429,7 +432,7 @@ def bh_main(args, pos, vel, eps, dt, what, metric, data, energy_theta=None, cpu=
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32+f16 far field" if args.far16 else "f32",
         "data": data,
         "config": {"workload": f"{what}, Barnes-Hut LBVH rebuilt every step, "
                                f"{args.key_bits}-bit keys, theta={args.theta}, eps={eps}, dt={dt}, "

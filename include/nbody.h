@@ -41,9 +41,18 @@ extern "C" {
  * (16, 32 = the reference's warp, or 64 = one wave64). */
 #define NB_TRAVERSE_THREAD 0
 #define NB_TRAVERSE_WARP 1
-/* accumulation of per-particle sums across tiles (all-pairs) */
+/* arithmetic of the force sums:
+ *   FP32          plain FP32 accumulation (the reference's);
+ *   COMPENSATED   Kahan-added tile (all-pairs) / 64-source block (wave64 walk)
+ *                 partial sums;
+ *   FP16_FAR      wave64 walk only: accepted cells evaluated in packed FP16
+ *                 (two interactions per instruction) relative to the group
+ *                 centre with masses scaled by 512, products accumulated in
+ *                 FP32; leaves and cells outside the safe FP16 range stay FP32.
+ *                 Needs force BH, traverse WARP, group_width 64, eps >= 0.008. */
 #define NB_PREC_FP32 0
 #define NB_PREC_COMPENSATED 1
+#define NB_PREC_FP16_FAR 2
 
 /* SimParams, src/simulation.hpp:8-15, plus the additive knobs. */
 typedef struct nb_params {

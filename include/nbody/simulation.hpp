@@ -19,6 +19,7 @@ struct SimParams {
   int key_bits = 63;                            // Morton key width: 63 or 30
   int group_width = 32;                         // lanes per kWarp group: 16, 32, 64
   bool compensated = false;                     // Kahan tile sums (all-pairs)
+  bool far16 = false;                           // FP16 far field (BH wave64 walk)
 };
 
 // Device-resident SoA state (simulation.hpp:17-28): pos = (x, y, z, m),

@@ -24,6 +24,7 @@ TRAVERSE_THREAD = 0
 TRAVERSE_WARP = 1
 PREC_FP32 = 0
 PREC_COMPENSATED = 1
+PREC_FP16_FAR = 2
 
 
 class NbParams(ctypes.Structure):

@@ -76,7 +76,14 @@ void validate(const Params& p) {
   if (p.key_bits != 30 && p.key_bits != 63) throw Error(kErrInvalid, "key_bits must be 30 or 63");
   if (p.group_width != 16 && p.group_width != 32 && p.group_width != 64)
     throw Error(kErrInvalid, "group_width must be 16, 32 or 64");
-  if (p.precision != 0 && p.precision != 1) throw Error(kErrInvalid, "precision must be 0 or 1");
+  if (p.precision < 0 || p.precision > 2) throw Error(kErrInvalid, "precision must be 0, 1 or 2");
+  if (p.precision == 2 && !(p.force == 1 && p.traverse == 1 && p.group_width == 64))
+    throw Error(kErrInvalid,
+                "precision 2 (FP16 far field) applies to the Barnes-Hut wave64 walk "
+                "(force=bh, traverse=warp, group_width=64)");
+  if (p.precision == 2 && !(p.eps >= 0.008f))
+    throw Error(kErrInvalid, "precision 2 (FP16 far field) needs epsilon >= 0.008 "
+                             "(eps^2 must be a normal FP16 number)");
   if (!(p.theta >= 0.f)) throw Error(kErrInvalid, "theta must be >= 0");
   if (!(p.eps >= 0.f)) throw Error(kErrInvalid, "epsilon must be >= 0");
 }
@@ -178,6 +185,7 @@ class Engine {
       w.overflow = &state.p->overflow;
       w.error = &state.p->error;
       w.compensated = p.precision == 1;
+      w.half_far = p.precision == 2;
       launch_walk(stream, p.traverse, p.group_width, w);
       mark(ce, kPhWalk + 1);
       tree_valid = true;
@@ -238,6 +246,7 @@ class Engine {
     w.overflow = &state.p->overflow;
     w.error = &state.p->error;
     w.compensated = p.precision == 1;
+    w.half_far = p.precision == 2;
     return w;
   }
 
@@ -1218,6 +1227,7 @@ int nb_traverse_counts(nb_engine* h, int traverse, int group_width, int32_t* int
     nb::WalkArgs w = e.walk_args();
     w.acc = nullptr;
     w.compensated = false;
+    w.half_far = false;
     w.counts = ci.p;
     w.opens = co.p;
     if (e.nsys > 0) {
@@ -1391,7 +1401,7 @@ nb::Params to_params(const SimParams& sp) {
   p.traverse = sp.traverse == BhTraversal::kWarp ? 1 : 0;
   p.key_bits = sp.key_bits;
   p.group_width = sp.group_width;
-  p.precision = sp.compensated ? 1 : 0;
+  p.precision = sp.far16 ? 2 : (sp.compensated ? 1 : 0);
   return p;
 }
 
@@ -1400,6 +1410,7 @@ void run_forces(const ParticleSystem& sys, float4* d_acc, const SimParams& sp, i
   nb::Engine& e = global_engine();
   nb::Params p = to_params(sp);
   p.force = force;
+  if (force == 0 && p.precision == 2) p.precision = 0;  // the FP16 far field is a walk mode
   e.set_params(p);
   e.bind(sys.d_pos, sys.d_vel, d_acc, sys.n);
   e.timing_on = timed;

@@ -6,7 +6,7 @@
 // --traverse thread|warp), same ic.bin / frame / energy CSV formats
 // (docs/FORMATS.md; main.cu:114-179) and the same stdout lines, so scripts
 // that parse the reference's output (gpu_bench.sh:31,41-44) work unchanged.
-// Additive flags: --keys 30|63, --group-width 16|32|64, --compensated.
+// Additive flags: --keys 30|63, --group-width 16|32|64, --compensated, --far16.
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -100,6 +100,8 @@ Args parse_args(int argc, char** argv) {
       else die("--group-width must be 16, 32 or 64, got: " + v);
     } else if (k == "--compensated") {
       a.params.compensated = true;
+    } else if (k == "--far16") {
+      a.params.far16 = true;
     } else {
       die("unknown argument: " + k);
     }
@@ -107,7 +109,8 @@ Args parse_args(int argc, char** argv) {
   if (a.ic_path.empty())
     die("usage: galaxy_sim --ic ic.bin [--steps N] [--dt f] [--eps f] [--dump-every N] "
         "[--out dir] [--force naive|bh] [--theta f] [--traverse thread|warp] "
-        "[--compare-forces] [--keys 30|63] [--group-width 16|32|64] [--compensated]");
+        "[--compare-forces] [--keys 30|63] [--group-width 16|32|64] [--compensated] "
+        "[--far16]");
   if (a.dump_every <= 0) die("--dump-every must be positive");
   return a;
 }

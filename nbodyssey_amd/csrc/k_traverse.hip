@@ -174,17 +174,67 @@ __device__ __forceinline__ void flush_list(const float4* list, int cnt, float bx
   }
 }
 
+// FP16 far field (precision mode 2).  Accepted cells whose magnitudes are
+// safely inside FP16 range are evaluated two per lane-instruction with packed
+// FP16 arithmetic, positions relative to the group's box centre and masses
+// scaled by 512; every product is accumulated in FP32 (mixed FMA) and scaled
+// back once.  A cell goes to this list only if |cell - centre| < 16384 per
+// axis and m / (dmin^2 + eps^2) <= 32 (dmin = distance to the group box), so
+// m * 512 * inv^2 <= 16384 < 65504 for every lane.
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+constexpr float kHalfMassScale = 512.f;
+constexpr float kHalfMaxAccel = 32.f;
+constexpr float kHalfMaxCoord = 16384.f;
+constexpr int kHalfList = 128;   // FP16 list entries per wave (64 pairs)
+
+// l16: pairs of entries, pair p = half2 (x0,x1), (y0,y1), (z0,z1), (m0,m1) at
+// l16[4p..4p+3]; npairs pairs.
+__device__ __forceinline__ void flush_half(const half2_t* l16, int npairs, half2_t bx, half2_t by,
+                                           half2_t bz, half2_t eps2h, float& fx, float& fy,
+                                           float& fz) {
+#pragma unroll 4
+  for (int p = 0; p < npairs; ++p) {
+    const half2_t X = l16[4 * p], Y = l16[4 * p + 1], Z = l16[4 * p + 2], M = l16[4 * p + 3];
+    const half2_t dx = X - bx, dy = Y - by, dz = Z - bz;
+    const half2_t r2 = __builtin_elementwise_fma(
+        dz, dz, __builtin_elementwise_fma(dy, dy, __builtin_elementwise_fma(dx, dx, eps2h)));
+    half2_t inv;
+    inv.x = __builtin_amdgcn_rsqh(r2.x);
+    inv.y = __builtin_amdgcn_rsqh(r2.y);
+    const half2_t w = M * (inv * inv);
+    const half2_t ux = dx * inv, uy = dy * inv, uz = dz * inv;
+    fx = fmaf((float)ux.x, (float)w.x, fx);
+    fy = fmaf((float)uy.x, (float)w.x, fy);
+    fz = fmaf((float)uz.x, (float)w.x, fz);
+    fx = fmaf((float)ux.y, (float)w.y, fx);
+    fy = fmaf((float)uy.y, (float)w.y, fy);
+    fz = fmaf((float)uz.y, (float)w.y, fz);
+  }
+}
+
 // One group of `count` (<= 64) consecutive sorted targets starting at t0,
 // walking the subtree under `root`, on one wave; `slot` indexes the count
 // outputs.  Dead lanes (lane >= count) repeat the group's last target, so the
 // group box and the representatives are those of the live targets.
-template <bool COUNT, bool COMP, bool POT = false>
+// MODE 0: FP32; 1: Kahan-compensated block sums; 2: FP16 far field.
+template <int MODE>
+struct WalkMode {
+  static constexpr bool comp = MODE == 1;
+  static constexpr bool half = MODE == 2;
+  static constexpr int stack = half ? 768 : kWaveStack;   // LDS stack entries per wave
+  static constexpr int l16 = half ? 2 * kHalfList : 1;     // half2 slots of the FP16 list
+};
+
+template <bool COUNT, int MODE, bool POT = false>
 __device__ __forceinline__ void walk_group64(
     int t0, int count, int root, int slot, const float4* __restrict__ pos_sorted,
     const unsigned* __restrict__ perm, int n, const Node* __restrict__ nodes, float theta2,
     float eps2, float G, float4* __restrict__ acc, float4* __restrict__ vel, float half_dt,
     int* __restrict__ counts, int* __restrict__ opens, int* overflow, int* error, int* stk,
-    float4* list, float* __restrict__ phi = nullptr) {
+    float4* list, half2_t* l16 = nullptr, float* __restrict__ phi = nullptr) {
+  constexpr bool COMP = WalkMode<MODE>::comp;
+  constexpr bool HALF = WalkMode<MODE>::half;
+  constexpr int WSTACK = WalkMode<MODE>::stack;
   const int lane = threadIdx.x & 63;
   const bool live = lane < count;
   const int t = t0 + (live ? lane : count - 1);
@@ -198,6 +248,17 @@ __device__ __forceinline__ void walk_group64(
   float bhx = wave_max(b.x), bhy = wave_max(b.y), bhz = wave_max(b.z);
   const float acc_lo = theta2 * (1.f - kMacMargin);
   const float acc_hi = theta2 * (1.f + kMacMargin);
+  // FP16 far field: group-relative frame
+  const float gcx = 0.5f * (blx + bhx), gcy = 0.5f * (bly + bhy), gcz = 0.5f * (blz + bhz);
+  half2_t hbx = {}, hby = {}, hbz = {}, heps2 = {};
+  float fx = 0.f, fy = 0.f, fz = 0.f;
+  int n16 = 0;
+  if (HALF) {
+    hbx = (half2_t)((_Float16)(b.x - gcx));
+    hby = (half2_t)((_Float16)(b.y - gcy));
+    hbz = (half2_t)((_Float16)(b.z - gcz));
+    heps2 = (half2_t)((_Float16)eps2);
+  }
   // representative targets spread over the group (Morton order spreads them
   // in space); if one of them opens a node by its own canonical test, the
   // node is open -- an exact shortcut, no margin needed.  A dead lane holds
@@ -222,7 +283,7 @@ __device__ __forceinline__ void walk_group64(
     // time (a depth-first walk needs at most one entry per tree level, and
     // the reserve exceeds the deepest tree), so the stack cannot overflow
     int k = sp < 64 ? sp : 64;
-    if (sp > kWaveStack - kStackReserve) k = 1;
+    if (sp > WSTACK - kStackReserve) k = 1;
     // pop k entries from the top: lane l takes stk[sp-1-l]
     const bool valid = lane < k;
     const int id = valid ? stk[sp - 1 - lane] : 0;
@@ -246,7 +307,7 @@ __device__ __forceinline__ void walk_group64(
       left = nd.left;
       right = nd.right;
     }
-    bool sure_open = false, ambiguous = false;
+    bool sure_open = false, ambiguous = false, safe16 = false;
     if (is_node) {
       const float ex = fmaxf(fmaxf(blx - src.x, src.x - bhx), 0.f);
       const float ey = fmaxf(fmaxf(bly - src.y, src.y - bhy), 0.f);
@@ -265,6 +326,10 @@ __device__ __forceinline__ void walk_group64(
         rmin = fminf(rmin, dist2(src.x - rx[r], src.y - ry[r], src.z - rz[r]));
       sure_open = (s2 > acc_hi * dmax2) | !(s2 < theta2 * rmin);
       ambiguous = !sure_accept & !sure_open;
+      if (HALF)
+        safe16 = (src.w <= kHalfMaxAccel * (dmin2 + eps2)) &
+                 (fabsf(src.x - gcx) < kHalfMaxCoord) & (fabsf(src.y - gcy) < kHalfMaxCoord) &
+                 (fabsf(src.z - gcz) < kHalfMaxCoord);
     }
     unsigned long long open_mask = __builtin_amdgcn_ballot_w64(sure_open);
     unsigned long long amb = __builtin_amdgcn_ballot_w64(ambiguous);
@@ -289,15 +354,19 @@ __device__ __forceinline__ void walk_group64(
       open_mask |= __builtin_amdgcn_ballot_w64(ambiguous && ((opened >> j) & 1ull));
     }
     const unsigned long long node_mask = __builtin_amdgcn_ballot_w64(is_node);
-    const unsigned long long src_mask =
+    const unsigned long long all_src =
         (node_mask & ~open_mask) | __builtin_amdgcn_ballot_w64(is_leaf);
+    // FP16 far field: accepted cells in range; everything else stays FP32
+    const unsigned long long far_mask =
+        HALF ? (node_mask & ~open_mask & __builtin_amdgcn_ballot_w64(safe16)) : 0ull;
+    const unsigned long long src_mask = all_src & ~far_mask;
     if (COUNT) {
-      ni += __popcll(src_mask);
+      ni += __popcll(all_src);
       no += __popcll(node_mask);
     }
     // push the children of opened nodes (lane order)
     const int nopen = __popcll(open_mask);
-    if (sp + 2 * nopen > kWaveStack) {
+    if (sp + 2 * nopen > WSTACK) {
       if (lane == 0) *overflow = 1;
       break;
     }
@@ -310,6 +379,24 @@ __device__ __forceinline__ void walk_group64(
     // append sources to the interaction list (lane order)
     if ((src_mask >> lane) & 1ull) list[nlist + __popcll(src_mask & lt)] = src;
     nlist += __popcll(src_mask);
+    if (HALF) {
+      if ((far_mask >> lane) & 1ull) {
+        const int j = n16 + __popcll(far_mask & lt);
+        _Float16* e = reinterpret_cast<_Float16*>(l16 + 4 * (j >> 1)) + (j & 1);
+        e[0] = (_Float16)(src.x - gcx);
+        e[2] = (_Float16)(src.y - gcy);
+        e[4] = (_Float16)(src.z - gcz);
+        e[6] = (_Float16)(src.w * kHalfMassScale);
+      }
+      n16 += __popcll(far_mask);
+      if (n16 >= 64) {
+        flush_half(l16, 32, hbx, hby, hbz, heps2, fx, fy, fz);
+        n16 -= 64;
+        // move the (< 64) remaining entries, pairs 32.., to the front
+        l16[lane] = l16[128 + lane];
+        l16[64 + lane] = l16[192 + lane];
+      }
+    }
     if (nlist >= 64) {
       flush_list<COMP, POT>(list, 64, b.x, b.y, b.z, eps2, ax, ay, az, cx, cy, cz, pot);
       nlist -= 64;
@@ -318,6 +405,16 @@ __device__ __forceinline__ void walk_group64(
   }
   if (nlist > 0)
     flush_list<COMP, POT>(list, nlist, b.x, b.y, b.z, eps2, ax, ay, az, cx, cy, cz, pot);
+  if (HALF) {
+    if (n16 & 1) {  // pad the last pair with a massless entry
+      if (lane < 4) reinterpret_cast<_Float16*>(l16 + 4 * (n16 >> 1))[2 * lane + 1] = 0;
+    }
+    if (n16 > 0) flush_half(l16, (n16 + 1) >> 1, hbx, hby, hbz, heps2, fx, fy, fz);
+    constexpr float inv_scale = 1.f / kHalfMassScale;
+    ax = fmaf(fx, inv_scale, ax);
+    ay = fmaf(fy, inv_scale, ay);
+    az = fmaf(fz, inv_scale, az);
+  }
   if (COUNT && lane == 0) {
     counts[slot] = (int)ni;
     opens[slot] = (int)no;
@@ -340,9 +437,9 @@ __global__ __launch_bounds__(kBlock) void walk_potential_kernel(
   const int w = threadIdx.x >> 6;
   const int gbase = xcd_block() * blockDim.x + (threadIdx.x & ~63);
   if (gbase >= n) return;  // wave-uniform
-  walk_group64<false, false, true>(gbase, min(64, n - gbase), 0, gbase >> 6, pos_sorted, perm, n,
-                                   nodes, theta2, eps2, 1.f, nullptr, nullptr, 0.f, nullptr,
-                                   nullptr, overflow, error, s_stack[w], s_list[w], phi);
+  walk_group64<false, 0, true>(gbase, min(64, n - gbase), 0, gbase >> 6, pos_sorted, perm, n,
+                               nodes, theta2, eps2, 1.f, nullptr, nullptr, 0.f, nullptr, nullptr,
+                               overflow, error, s_stack[w], s_list[w], nullptr, phi);
 }
 
 // Energy from the potential walk, FP64 per block: KE = sum m v^2 / 2,
@@ -412,28 +509,29 @@ __global__ __launch_bounds__(kBlock) void tree_energy_final_kernel(
   }
 }
 
-template <bool COUNT, bool COMP>
+template <bool COUNT, int MODE>
 __global__ __launch_bounds__(kBlock) void walk_wave_kernel(
     const float4* __restrict__ pos_sorted, const unsigned* __restrict__ perm, int n,
     const Node* __restrict__ nodes, float theta2, float eps2, float G, float4* __restrict__ acc,
     float4* __restrict__ vel, float half_dt, int* __restrict__ counts, int* __restrict__ opens,
     int* overflow, int* error) {
   constexpr int WAVES = kBlock / 64;
-  __shared__ int s_stack[WAVES][kWaveStack];
+  __shared__ int s_stack[WAVES][WalkMode<MODE>::stack];
   __shared__ float4 s_list[WAVES][kWaveList];
+  __shared__ half2_t s_l16[WAVES][WalkMode<MODE>::l16];
   const int w = threadIdx.x >> 6;
   const int gbase = xcd_block() * blockDim.x + (threadIdx.x & ~63);
   if (gbase >= n) return;  // wave-uniform
-  walk_group64<COUNT, COMP>(gbase, min(64, n - gbase), 0, gbase >> 6, pos_sorted, perm, n, nodes,
+  walk_group64<COUNT, MODE>(gbase, min(64, n - gbase), 0, gbase >> 6, pos_sorted, perm, n, nodes,
                             theta2, eps2, G, acc, vel, half_dt, counts, opens, overflow, error,
-                            s_stack[w], s_list[w]);
+                            s_stack[w], s_list[w], s_l16[w]);
 }
 
 // Persistent ensemble walk: each wave pulls work items (system, first sorted
 // target, count <= 64) from a device queue until the queue is empty, walking
 // the system's own subtree (root = roots[system]) of the forest.  Every wave
 // exits once the counter passes the item count.
-template <bool COUNT, bool COMP>
+template <bool COUNT, int MODE>
 __global__ __launch_bounds__(kBlock) void walk_items_kernel(
     const float4* __restrict__ pos_sorted, const unsigned* __restrict__ perm, int n,
     const Node* __restrict__ nodes, float theta2, float eps2, float G, float4* __restrict__ acc,
@@ -441,8 +539,9 @@ __global__ __launch_bounds__(kBlock) void walk_items_kernel(
     int* overflow, int* error, const int4* __restrict__ items, int nitems,
     const int* __restrict__ roots, unsigned* queue) {
   constexpr int WAVES = kBlock / 64;
-  __shared__ int s_stack[WAVES][kWaveStack];
+  __shared__ int s_stack[WAVES][WalkMode<MODE>::stack];
   __shared__ float4 s_list[WAVES][kWaveList];
+  __shared__ half2_t s_l16[WAVES][WalkMode<MODE>::l16];
   const int w = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   while (true) {
@@ -451,9 +550,9 @@ __global__ __launch_bounds__(kBlock) void walk_items_kernel(
     k = __builtin_amdgcn_readfirstlane(k);
     if (k >= (unsigned)nitems) break;
     const int4 it = items[k];  // (system, first target, count, -)
-    walk_group64<COUNT, COMP>(it.y, it.z, roots[it.x], (int)k, pos_sorted, perm, n, nodes, theta2,
+    walk_group64<COUNT, MODE>(it.y, it.z, roots[it.x], (int)k, pos_sorted, perm, n, nodes, theta2,
                               eps2, G, acc, vel, half_dt, counts, opens, overflow, error,
-                              s_stack[w], s_list[w]);
+                              s_stack[w], s_list[w], s_l16[w]);
   }
 }
 
@@ -534,10 +633,12 @@ void walk_dispatch(hipStream_t s, int traverse, int width, const WalkArgs& a) {
   if (traverse == 0) {
     walk_thread_kernel<COUNT><<<g, kBlock, 0, s>>>(NB_WALK_ARGS);
   } else if (width == 64) {
-    if (a.compensated)
-      walk_wave_kernel<COUNT, true><<<g, kBlock, 0, s>>>(NB_WALK_ARGS);
+    if (a.half_far && !COUNT)
+      walk_wave_kernel<COUNT, 2><<<g, kBlock, 0, s>>>(NB_WALK_ARGS);
+    else if (a.compensated && !COUNT)
+      walk_wave_kernel<COUNT, 1><<<g, kBlock, 0, s>>>(NB_WALK_ARGS);
     else
-      walk_wave_kernel<COUNT, false><<<g, kBlock, 0, s>>>(NB_WALK_ARGS);
+      walk_wave_kernel<COUNT, 0><<<g, kBlock, 0, s>>>(NB_WALK_ARGS);
   } else if (width == 32) {
     walk_group_kernel<32, COUNT><<<g, kBlock, 0, s>>>(NB_WALK_ARGS);
   } else if (width == 16) {
@@ -559,7 +660,7 @@ static int walk_items_grid() {
     NB_HIP(hipGetDevice(&dev));
     NB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     NB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, reinterpret_cast<const void*>(walk_items_kernel<false, false>), kBlock, 0));
+        &per_cu, reinterpret_cast<const void*>(walk_items_kernel<false, 0>), kBlock, 0));
     grid = std::max(1, cus) * std::max(1, per_cu);
   }
   return grid;
@@ -577,11 +678,13 @@ void launch_walk_items(hipStream_t s, const WalkArgs& a, const int4* items, int 
   a.pos_sorted, a.perm, a.n, a.nodes, a.theta2, a.eps2, a.G, a.acc, a.vel, a.half_dt, a.counts, \
       a.opens, a.overflow, a.error, items, nitems, roots, queue
   if (a.counts)
-    walk_items_kernel<true, false><<<grid, kBlock, 0, s>>>(NB_ITEM_ARGS);
+    walk_items_kernel<true, 0><<<grid, kBlock, 0, s>>>(NB_ITEM_ARGS);
+  else if (a.half_far)
+    walk_items_kernel<false, 2><<<grid, kBlock, 0, s>>>(NB_ITEM_ARGS);
   else if (a.compensated)
-    walk_items_kernel<false, true><<<grid, kBlock, 0, s>>>(NB_ITEM_ARGS);
+    walk_items_kernel<false, 1><<<grid, kBlock, 0, s>>>(NB_ITEM_ARGS);
   else
-    walk_items_kernel<false, false><<<grid, kBlock, 0, s>>>(NB_ITEM_ARGS);
+    walk_items_kernel<false, 0><<<grid, kBlock, 0, s>>>(NB_ITEM_ARGS);
 #undef NB_ITEM_ARGS
   NB_LAUNCH_CHECK();
 }

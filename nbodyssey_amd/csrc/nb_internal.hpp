@@ -160,6 +160,7 @@ struct WalkArgs {
   int* overflow;      // set to 1 if a stack would overflow
   int* error;         // set to 1 if a node id is out of range or the walk is too long
   bool compensated;   // Kahan-add each block of interactions (wave64 walk)
+  bool half_far;      // FP16 far field for accepted cells (wave64 walk)
 };
 void launch_walk(hipStream_t s, int traverse, int group_width, const WalkArgs& a);
 // Energy from a potential walk at theta2 (< 1/3) on the built tree:

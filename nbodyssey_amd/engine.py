@@ -31,7 +31,8 @@ class Simulation:
     integrator.cuh)."""
 
     def __init__(self, pos4, vel4=None, dt=0.01, eps=0.05, G=1.0, theta=0.5, force="naive",
-                 traverse="thread", key_bits=63, group_width=32, compensated=False):
+                 traverse="thread", key_bits=63, group_width=32, compensated=False,
+                 far16=False):
         L = _lib.load()
         pos4 = as_f4(pos4)
         self.n = len(pos4)
@@ -40,7 +41,8 @@ class Simulation:
         self._h = h
         self._L = L
         self.set_params(dt=dt, eps=eps, G=G, theta=theta, force=force, traverse=traverse,
-                        key_bits=key_bits, group_width=group_width, compensated=compensated)
+                        key_bits=key_bits, group_width=group_width, compensated=compensated,
+                        far16=far16)
         self.upload(pos4, vel4)
 
     # -- lifetime
@@ -63,17 +65,19 @@ class Simulation:
 
     # -- parameters
     def set_params(self, dt=0.01, eps=0.05, G=1.0, theta=0.5, force="naive", traverse="thread",
-                   key_bits=63, group_width=32, compensated=False):
+                   key_bits=63, group_width=32, compensated=False, far16=False):
         p = NbParams()
         p.dt, p.epsilon, p.G, p.theta = dt, eps, G, theta
         p.force = FORCES[force] if isinstance(force, str) else int(force)
         p.traverse = TRAVERSALS[traverse] if isinstance(traverse, str) else int(traverse)
         p.key_bits = key_bits
         p.group_width = group_width
-        p.precision = _lib.PREC_COMPENSATED if compensated else _lib.PREC_FP32
+        p.precision = (_lib.PREC_FP16_FAR if far16 else
+                       _lib.PREC_COMPENSATED if compensated else _lib.PREC_FP32)
         check(self._L.nb_set_params(self._h, ctypes.byref(p)))
         self.params = dict(dt=dt, eps=eps, G=G, theta=theta, force=force, traverse=traverse,
-                           key_bits=key_bits, group_width=group_width, compensated=compensated)
+                           key_bits=key_bits, group_width=group_width, compensated=compensated,
+                           far16=far16)
 
     def update(self, **kw):
         p = dict(self.params)
@@ -212,7 +216,8 @@ class Ensemble(Simulation):
     (system, 30-bit Morton) keys; dt, eps, G, theta apply to every system.
     """
 
-    def __init__(self, systems, dt=0.01, eps=0.05, G=1.0, theta=0.5, compensated=False):
+    def __init__(self, systems, dt=0.01, eps=0.05, G=1.0, theta=0.5, compensated=False,
+                 far16=False):
         L = _lib.load()
         pos = [as_f4(p) for p, _ in systems]
         vel = [np.zeros_like(p) if v is None else as_f4(v, len(p))
@@ -227,7 +232,7 @@ class Ensemble(Simulation):
         self.offsets = np.zeros(self.nsys + 1, np.int32)
         check(L.nb_ensemble_offsets(self._h, ptr(self.offsets)))
         self.set_params(dt=dt, eps=eps, G=G, theta=theta, force="bh", traverse="warp",
-                        key_bits=30, group_width=64, compensated=compensated)
+                        key_bits=30, group_width=64, compensated=compensated, far16=far16)
         self.upload(np.concatenate(pos), np.concatenate(vel))
 
     def split(self, a):

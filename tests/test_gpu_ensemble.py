@@ -171,3 +171,16 @@ def test_direct_f64_per_system():
         mine = (idx >= off[s]) & (idx < off[s + 1])
         ref = orc.allpairs_f64(pos, eps, targets=(idx[mine] - off[s]).astype(np.int32))
         np.testing.assert_allclose(got[mine], ref, rtol=1e-12, atol=1e-14 * np.abs(ref).max())
+
+
+def test_ensemble_far16():
+    """FP16 far field in the persistent ensemble walk: same per-system result
+    as the standalone FP16 walk, bitwise."""
+    systems, _ = ic.ensemble_encounters(8, seed=400, n_min=2048, n_max=8192)
+    with engine.Ensemble(systems, eps=EPS, theta=0.5, far16=True) as ens:
+        acc = ens.compute_forces()
+        off = ens.offsets
+    for s, (pos, _) in enumerate(systems):
+        with engine.Simulation(pos, eps=EPS, theta=0.5, force="bh", traverse="warp",
+                               group_width=64, key_bits=30, far16=True) as sim:
+            assert np.array_equal(acc[off[s]:off[s + 1]], sim.compute_forces()), s

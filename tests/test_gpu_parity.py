@@ -418,3 +418,52 @@ def test_autotune_ensemble():
         assert np.array_equal(ens.download()[0], p0)
         assert choice["walk_grid"] > 0 and choice["median_err"] <= 0.01
         ens.step(1)
+
+
+def test_far16_theta0_is_fp32():
+    """With theta = 0 no cell is accepted, so the FP16 far field has nothing
+    to do: the result is the FP32 walk's up to summation order (the FP16 mode
+    keeps a smaller LDS stack, so its batches are cut differently)."""
+    pos, _ = ic.two_disk_encounter(20000, seed=51, eps=0.02)
+    kw = dict(eps=0.02, theta=0.0, force="bh", traverse="warp", group_width=64, key_bits=30)
+    with engine.Simulation(pos, **kw) as a, engine.Simulation(pos, far16=True, **kw) as b:
+        d = rel_err(b.compute_forces(), a.compute_forces())
+    assert np.median(d) < 1e-6 and d.max() < 1e-4, (np.median(d), d.max())
+
+
+@pytest.mark.parametrize("theta", [0.5, 0.7])
+def test_far16_error_budget(theta):
+    """FP16 far field at 65,536 particles: the median / p99 force error vs FP64
+    stays within 5e-4 / 2e-3 (absolute) of the FP32 walk's -- well under the
+    multipole error (~0.7% median at theta 0.5) -- and the interaction counts
+    are unchanged (same node decisions)."""
+    pos, _ = ic.two_disk_encounter(65536, seed=1, eps=0.02)
+    kw = dict(eps=0.02, theta=theta, force="bh", traverse="warp", group_width=64, key_bits=30)
+    idx = np.random.default_rng(0).choice(len(pos), 512, replace=False).astype(np.int32)
+    with engine.Simulation(pos, **kw) as a, engine.Simulation(pos, far16=True, **kw) as b:
+        a32 = a.compute_forces()
+        a16 = b.compute_forces()
+        ref = a.direct_f64(idx)
+        ca = a.traverse_counts("warp", 64)
+        cb = b.traverse_counts("warp", 64)
+    e32 = np.linalg.norm(a32[idx, :3] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    e16 = np.linalg.norm(a16[idx, :3] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    d = rel_err(a16, a32)
+    print(f"far16 theta={theta}: median fp32 {np.median(e32):.3e} fp16 {np.median(e16):.3e}; "
+          f"p99 {np.quantile(e32, .99):.3e} / {np.quantile(e16, .99):.3e}; "
+          f"fp16 vs fp32 median {np.median(d):.2e} max {d.max():.2e}", flush=True)
+    assert np.array_equal(ca[0], cb[0]) and np.array_equal(ca[1], cb[1])
+    assert np.median(e16) < np.median(e32) + 5e-4
+    assert np.quantile(e16, 0.99) < np.quantile(e32, 0.99) + 2e-3
+    assert np.all(np.isfinite(a16))
+
+
+def test_far16_parameter_checks():
+    pos, _ = ic.two_disk_encounter(1000, seed=3, eps=0.05)
+    with pytest.raises(NbError):
+        engine.Simulation(pos, force="naive", far16=True)
+    with pytest.raises(NbError):
+        engine.Simulation(pos, force="bh", traverse="warp", group_width=32, far16=True)
+    with pytest.raises(NbError):
+        engine.Simulation(pos, eps=0.005, force="bh", traverse="warp", group_width=64,
+                          far16=True)
