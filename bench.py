@@ -224,8 +224,8 @@ def ensemble_main(args):
 
 def precision_main(args):
     """Precision study on one two-disk system (default 4,194,304 particles,
-    eps 0.02, dt 0.005): for theta in {0.3, 0.5, 0.7} and the FP32 and
-    compensated (Kahan) walks, the step time, the force error on a seeded
+    eps 0.02, dt 0.005): for theta in {0.3, 0.5, 0.7} and the FP32, compensated
+    (Kahan) and FP16-far-field walks, the step time, the force error on a seeded
     subsample against FP64 direct summation on the GPU, and the relative
     total-energy drift over `drift_steps` steps (FP64 energy on the GPU)."""
     from nbodyssey_amd import engine, ic
@@ -238,10 +238,11 @@ def precision_main(args):
     rows = []
     headline = None
     for theta in (0.3, 0.5, 0.7):
-        for comp in (False, True):
+        for form in ("fp32", "kahan", "fp16far"):
+            comp = form == "kahan"
             with engine.Simulation(pos, vel, dt=dt, eps=eps, theta=theta, force="bh",
                                    traverse="warp", group_width=64, key_bits=30,
-                                   compensated=comp) as sim:
+                                   compensated=comp, far16=form == "fp16far") as sim:
                 a = sim.compute_forces()
                 err = sim.force_error(idx, a)
                 sim.step(args.warmup)
@@ -255,14 +256,14 @@ def precision_main(args):
                     sim.step(args.drift_steps)
                     e1 = sim.total_energy()
                     drift = float((e1[0] - e0[0]) / abs(e0[0]))
-            row = {"theta": theta, "precision": "kahan" if comp else "fp32",
+            row = {"theta": theta, "precision": form,
                    "ms_per_step": ms / args.steps,
                    "median_rel_force_err_vs_fp64": float(np.median(err)),
                    "p99_rel_force_err_vs_fp64": float(np.quantile(err, 0.99)),
                    "energy_drift": drift, "drift_steps": args.drift_steps}
             rows.append(row)
             print(f"bench: {row}", file=sys.stderr, flush=True)
-            if theta == 0.5 and not comp:
+            if theta == 0.5 and form == "fp32":
                 headline = row
     value = n / (headline["ms_per_step"] * 1e-3)
     out = {

@@ -16,3 +16,7 @@ void leapfrog_step(ParticleSystem& sys, float4* d_acc, const SimParams& p);
 
 // KE + PE reduced in FP64 on the device (integrator.cu:93-110).
 Energy total_energy(const ParticleSystem& sys, const SimParams& p);
+
+// Additive: the same energy at O(n log n) cost for large n -- PE from a
+// wave64 tree walk at theta_e in (0, 0.577) (see nb_total_energy_tree).
+Energy total_energy_tree(const ParticleSystem& sys, const SimParams& p, float theta_e);

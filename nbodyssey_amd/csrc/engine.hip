@@ -570,9 +570,22 @@ class Engine {
     if (!(p.eps > 0.f)) throw Error(kErrInvalid, "tree energy needs epsilon > 0");
     if (n < 2) throw Error(kErrInvalid, "tree energy needs at least 2 particles");
     if (!tree_valid) {
-      const bool saved = timing_on;
+      // build the tree for the current positions; the accelerations of this
+      // pass go to scratch, so the caller's acceleration array is untouched
+      const bool saved = timing_on, saved_valid = acc_valid;
+      float4* saved_acc = acc;
+      if (scratch_acc.count < (size_t)n) scratch_acc.alloc((size_t)n);
+      acc = scratch_acc.p;
       timing_on = false;
-      force_pass(false);
+      try {
+        force_pass(false);
+      } catch (...) {
+        acc = saved_acc;
+        timing_on = saved;
+        throw;
+      }
+      acc = saved_acc;
+      acc_valid = saved_valid;
       timing_on = saved;
     }
     const int parts = tree_energy_partials(n);
@@ -750,6 +763,7 @@ class Engine {
 
   DevBuf<float4> own_pos, own_vel, own_acc;
   DevBuf<float> phi;
+  DevBuf<float4> scratch_acc;
   DevBuf<float4> ap_partial;
   DevBuf<double> en_partial, en_out;
   int bh_cap = 0, max_tiles = 0;
@@ -1515,6 +1529,27 @@ void leapfrog_step(ParticleSystem& sys, float4* d_acc, const SimParams& sp) {
     e.acc_valid = true;  // the caller's contract: d_acc holds a(x)
     e.force_pass(true);
   });
+}
+
+Energy total_energy_tree(const ParticleSystem& sys, const SimParams& sp, float theta_e) {
+  Energy out{0.0, 0.0, 0.0};
+  api("total_energy_tree", [&] {
+    nb::Engine& e = global_engine();
+    nb::Params p = to_params(sp);
+    p.force = 1;
+    if (p.traverse != 1 || p.group_width != 64 || p.precision == 2) {
+      p.traverse = 1;
+      p.group_width = 64;
+      if (p.precision == 2) p.precision = 0;
+    }
+    e.set_params(p);
+    e.bind(sys.d_pos, sys.d_vel, e.acc, sys.n);
+    e.tree_valid = false;  // the caller may have changed the positions
+    double r[3];
+    e.tree_energy(theta_e, r);
+    out = Energy{r[0], r[1], r[2]};
+  });
+  return out;
 }
 
 Energy total_energy(const ParticleSystem& sys, const SimParams& sp) {

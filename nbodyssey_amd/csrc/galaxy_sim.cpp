@@ -6,7 +6,9 @@
 // --traverse thread|warp), same ic.bin / frame / energy CSV formats
 // (docs/FORMATS.md; main.cu:114-179) and the same stdout lines, so scripts
 // that parse the reference's output (gpu_bench.sh:31,41-44) work unchanged.
-// Additive flags: --keys 30|63, --group-width 16|32|64, --compensated, --far16.
+// Additive flags: --keys 30|63, --group-width 16|32|64, --compensated, --far16,
+// --energy-theta f (energy log from the tree potential at that theta instead
+// of the O(n^2) sum: the cost policy for large n).
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -31,6 +33,7 @@ struct Args {
   int steps = 1000;
   int dump_every = 5;
   bool compare = false;
+  float energy_theta = 0.f;  // 0: exact O(n^2) energy
   SimParams params{/*dt=*/0.01f, /*epsilon=*/0.05f, /*G=*/1.0f};
 };
 
@@ -102,6 +105,10 @@ Args parse_args(int argc, char** argv) {
       a.params.compensated = true;
     } else if (k == "--far16") {
       a.params.far16 = true;
+    } else if (k == "--energy-theta") {
+      a.energy_theta = to_float(k, next());
+      if (!(a.energy_theta >= 0.f && a.energy_theta < 0.577f))
+        die("--energy-theta must be in [0, 0.577)");
     } else {
       die("unknown argument: " + k);
     }
@@ -110,7 +117,7 @@ Args parse_args(int argc, char** argv) {
     die("usage: galaxy_sim --ic ic.bin [--steps N] [--dt f] [--eps f] [--dump-every N] "
         "[--out dir] [--force naive|bh] [--theta f] [--traverse thread|warp] "
         "[--compare-forces] [--keys 30|63] [--group-width 16|32|64] [--compensated] "
-        "[--far16]");
+        "[--far16] [--energy-theta f]");
   if (a.dump_every <= 0) die("--dump-every must be positive");
   return a;
 }
@@ -197,7 +204,11 @@ int main(int argc, char** argv) {
   elog << "step,time,energy,kinetic,potential,rel_error\n";
 
   bool bh = args.params.force == ForceMethod::kBarnesHut;
-  Energy e0 = total_energy(sys, args.params);
+  auto energy = [&]() {
+    return args.energy_theta > 0.f ? total_energy_tree(sys, args.params, args.energy_theta)
+                                   : total_energy(sys, args.params);
+  };
+  Energy e0 = energy();
   std::printf("n=%d dt=%.4g eps=%.4g steps=%d force=%s", n, args.params.dt, args.params.epsilon,
               args.steps, bh ? "barnes-hut" : "naive");
   if (bh)
@@ -207,7 +218,7 @@ int main(int argc, char** argv) {
   std::printf("E0=%.6e KE0=%.6e PE0=%.6e\n", e0.total, e0.kinetic, e0.potential);
 
   auto log_energy = [&](int step) {
-    Energy e = total_energy(sys, args.params);
+    Energy e = energy();
     double rel = (e.total - e0.total) / std::abs(e0.total);
     char line[256];
     std::snprintf(line, sizeof(line), "%d,%.5f,%.8e,%.8e,%.8e,%.3e\n", step,

@@ -467,3 +467,27 @@ def test_far16_parameter_checks():
     with pytest.raises(NbError):
         engine.Simulation(pos, eps=0.005, force="bh", traverse="warp", group_width=64,
                           far16=True)
+
+
+def test_galaxy_sim_tree_energy_and_far16(tmp_path):
+    """Additive CLI flags: --energy-theta logs the tree-potential energy (E0
+    within 1e-3 of the exact log's), --far16 runs the FP16 far field; both
+    keep the reference's output format."""
+    import subprocess
+    from nbodyssey_amd import GALAXY_SIM, formats
+    pos, vel = ic.generate_ic_disks(particles=8000, seed=0)
+    icp = tmp_path / "ic.bin"
+    formats.write_ic(icp, pos, vel)
+    base = [GALAXY_SIM, "--ic", str(icp), "--steps", "10", "--dump-every", "5", "--out",
+            str(tmp_path / "f"), "--force", "bh", "--traverse", "warp", "--group-width", "64"]
+    r1 = subprocess.run(base + ["--energy-log", str(tmp_path / "exact.csv")],
+                        capture_output=True, text=True, timeout=120)
+    r2 = subprocess.run(base + ["--energy-log", str(tmp_path / "tree.csv"), "--energy-theta",
+                                "0.3", "--far16"], capture_output=True, text=True, timeout=120)
+    print(r1.stdout, r1.stderr, r2.stdout, r2.stderr, flush=True)
+    assert r1.returncode == 0 and r2.returncode == 0
+    ex = formats.read_energy_csv(tmp_path / "exact.csv")
+    tr = formats.read_energy_csv(tmp_path / "tree.csv")
+    assert list(ex["step"]) == list(tr["step"]) == [0, 5, 10]
+    assert np.all(np.abs(tr["energy"] / ex["energy"] - 1) < 1e-3)
+    assert np.all(np.abs(tr["rel_error"]) < 1e-3)
