@@ -133,7 +133,7 @@ __global__ __launch_bounds__(kBlock) void sort_init_kernel(SortState* ss, int pa
 constexpr unsigned long long kFlagAgg = 1ull << 32;
 constexpr unsigned long long kFlagInc = 2ull << 32;
 
-template <typename K, bool FIRST, bool GATHER>
+template <typename K, bool FIRST, bool GATHER, int ITEMS>
 __global__ __launch_bounds__(kBlock) void onesweep_kernel(
     const K* __restrict__ kin, const unsigned* __restrict__ vin, K* __restrict__ kout,
     unsigned* __restrict__ vout, int n, int pass, SortState* ss,
@@ -156,12 +156,12 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
     return;
   }
   const unsigned epoch = ss->epoch & 0x3fffffffu;
-  const int base = (int)tile * kSortTile + w * (64 * kSortItems);
+  const int base = (int)tile * (kBlock * ITEMS) + w * (64 * ITEMS);
 
-  K key[kSortItems];
-  unsigned val[kSortItems], rank[kSortItems];
+  K key[ITEMS];
+  unsigned val[ITEMS], rank[ITEMS];
 #pragma unroll
-  for (int j = 0; j < kSortItems; ++j) {
+  for (int j = 0; j < ITEMS; ++j) {
     int idx = base + j * 64 + lane;
     bool ok = idx < n;
     key[j] = ok ? kin[idx] : (K)~(K)0;
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
   }
   const unsigned long long lt = (1ull << lane) - 1ull;
 #pragma unroll
-  for (int j = 0; j < kSortItems; ++j) {
+  for (int j = 0; j < ITEMS; ++j) {
     unsigned d = (unsigned)(key[j] >> shift) & 255u;
     unsigned long long peers = ~0ull;
 #pragma unroll
@@ -227,7 +227,7 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
   __syncthreads();
 
 #pragma unroll
-  for (int j = 0; j < kSortItems; ++j) {
+  for (int j = 0; j < ITEMS; ++j) {
     int idx = base + j * 64 + lane;
     if (idx < n) {
       unsigned d = (unsigned)(key[j] >> shift) & 255u;
@@ -580,7 +580,12 @@ template <typename K>
 void sort_typed(hipStream_t s, K* ka, K* kb, unsigned* va, unsigned* vb, int n, SortState* ss,
                 unsigned long long* lookback, int max_tiles, bool first_identity,
                 const float4* gsrc, float4* gdst, int passes) {
-  int tiles = (n + kSortTile - 1) / kSortTile;
+  // large inputs sort faster with 6144-key tiles (fewer tiles and look-back
+  // steps; measured 4.2 vs 5.0 ms for 8 passes over 33.5M keys), smaller ones
+  // with 4096-key tiles (more workgroups to fill the chip)
+  const bool big = n >= (1 << 23);
+  const int tile_keys = kBlock * (big ? kSortItemsBig : kSortItems);
+  int tiles = (n + tile_keys - 1) / tile_keys;
   if (tiles > max_tiles) throw Error(kErrState, "sort look-back buffer too small");
   for (int p = 0; p < passes; ++p) {
     const K* kin = (p & 1) ? kb : ka;
@@ -589,18 +594,21 @@ void sort_typed(hipStream_t s, K* ka, K* kb, unsigned* va, unsigned* vb, int n, 
     unsigned* vout = (p & 1) ? va : vb;
     bool first = p == 0 && first_identity;
     bool gather = p == passes - 1 && gsrc != nullptr;
-    if (first && gather)
-      onesweep_kernel<K, true, true><<<tiles, kBlock, 0, s>>>(kin, vin, kout, vout, n, p, ss,
-                                                              lookback, max_tiles, gsrc, gdst);
-    else if (first)
-      onesweep_kernel<K, true, false><<<tiles, kBlock, 0, s>>>(kin, vin, kout, vout, n, p, ss,
-                                                               lookback, max_tiles, gsrc, gdst);
-    else if (gather)
-      onesweep_kernel<K, false, true><<<tiles, kBlock, 0, s>>>(kin, vin, kout, vout, n, p, ss,
-                                                               lookback, max_tiles, gsrc, gdst);
-    else
-      onesweep_kernel<K, false, false><<<tiles, kBlock, 0, s>>>(kin, vin, kout, vout, n, p, ss,
-                                                                lookback, max_tiles, gsrc, gdst);
+#define NB_SWEEP(F, G, I)                                                                  \
+  onesweep_kernel<K, F, G, I><<<tiles, kBlock, 0, s>>>(kin, vin, kout, vout, n, p, ss, lookback, \
+                                                       max_tiles, gsrc, gdst)
+    if (big) {
+      if (first && gather) NB_SWEEP(true, true, kSortItemsBig);
+      else if (first) NB_SWEEP(true, false, kSortItemsBig);
+      else if (gather) NB_SWEEP(false, true, kSortItemsBig);
+      else NB_SWEEP(false, false, kSortItemsBig);
+    } else {
+      if (first && gather) NB_SWEEP(true, true, kSortItems);
+      else if (first) NB_SWEEP(true, false, kSortItems);
+      else if (gather) NB_SWEEP(false, true, kSortItems);
+      else NB_SWEEP(false, false, kSortItems);
+    }
+#undef NB_SWEEP
     NB_LAUNCH_CHECK();
   }
 }

@@ -221,7 +221,7 @@ template <int MODE>
 struct WalkMode {
   static constexpr bool comp = MODE == 1;
   static constexpr bool half = MODE == 2;
-  static constexpr int stack = half ? 768 : kWaveStack;   // LDS stack entries per wave
+  static constexpr int stack = kWaveStack;                 // LDS stack entries per wave
   static constexpr int l16 = half ? 2 * kHalfList : 1;     // half2 slots of the FP16 list
 };
 

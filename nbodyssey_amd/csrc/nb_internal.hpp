@@ -49,7 +49,8 @@ constexpr int kBlock = 256;          // kForceBlock (forces.cuh:25)
 constexpr int kStackDepth = 128;     // kBhStack (forces_bh.cu:43)
 constexpr int kBboxBlocks = 1024;    // max partial boxes written by the bbox / kick-drift-bbox pass
 constexpr int kSortItems = 16;       // keys per thread per onesweep tile
-constexpr int kSortTile = kBlock * kSortItems;
+constexpr int kSortItemsBig = 24;    // the same for n >= 2^23
+constexpr int kSortTile = kBlock * kSortItems;   // smallest tile (sizes the look-back)
 constexpr int kMaxPasses = 8;
 
 // One internal tree node, 32 bytes so a wave-uniform walk fetches it with a

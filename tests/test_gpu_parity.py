@@ -491,3 +491,19 @@ def test_galaxy_sim_tree_energy_and_far16(tmp_path):
     assert list(ex["step"]) == list(tr["step"]) == [0, 5, 10]
     assert np.all(np.abs(tr["energy"] / ex["energy"] - 1) < 1e-3)
     assert np.all(np.abs(tr["rel_error"]) < 1e-3)
+
+
+@pytest.mark.parametrize("bits", [63, 30])
+def test_sort_big_tiles(bits):
+    """n >= 2^23 sorts with 6144-key tiles: still bit-exact (keys and stable
+    permutation) against a stable argsort, with heavy ties."""
+    n = (1 << 23) + 4097
+    rng = np.random.default_rng(bits)
+    hi = (1 << 30) if bits == 30 else (1 << 63)
+    uniq = rng.integers(0, hi, size=n // 5, dtype=np.uint64)
+    keys = uniq[rng.integers(0, len(uniq), n)]
+    print(f"big sort n={n} bits={bits}", flush=True)
+    ks, perm = engine.sort_pairs(keys, bits)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(perm.astype(np.int64), order)
+    assert np.array_equal(ks, keys[order])
