@@ -256,14 +256,23 @@ __global__ __launch_bounds__(kBlock) void direct_f64_kernel(const float4* __rest
   }
 }
 
-__global__ void direct_f64_reduce_kernel(const double* __restrict__ partial, int nt, int split,
-                                         double G, double* __restrict__ out) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nt) return;
+// One workgroup per target: thread t sums chunks t, t + 256, ... in order,
+// then a fixed-order tree over the threads (deterministic for a given split).
+__global__ __launch_bounds__(kBlock) void direct_f64_reduce_kernel(
+    const double* __restrict__ partial, int nt, int split, double G, double* __restrict__ out) {
+  __shared__ double s_a[3][kBlock];
+  const int k = blockIdx.x;
   double a[3] = {0.0, 0.0, 0.0};
-  for (int y = 0; y < split; ++y)
+  for (int y = threadIdx.x; y < split; y += kBlock)
     for (int c = 0; c < 3; ++c) a[c] += partial[((size_t)y * nt + k) * 3 + c];
-  for (int c = 0; c < 3; ++c) out[(size_t)k * 3 + c] = G * a[c];
+  for (int c = 0; c < 3; ++c) s_a[c][threadIdx.x] = a[c];
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int c = 0; c < 3; ++c) s_a[c][threadIdx.x] += s_a[c][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) out[(size_t)k * 3 + threadIdx.x] = G * s_a[threadIdx.x][0];
 }
 
 int split_for(int blocks_x, int n) {
@@ -339,8 +348,7 @@ void launch_direct_f64(hipStream_t s, const float4* pos, int lo, int hi, const i
   direct_f64_kernel<<<dim3(bx, ny), kBlock, 0, s>>>(pos, lo, hi, chunk, targets, nt, eps2,
                                                     partial);
   NB_LAUNCH_CHECK();
-  direct_f64_reduce_kernel<<<(nt + kBlock - 1) / kBlock, kBlock, 0, s>>>(partial, nt, ny, G,
-                                                                        out);
+  direct_f64_reduce_kernel<<<nt, kBlock, 0, s>>>(partial, nt, ny, G, out);
   NB_LAUNCH_CHECK();
 }
 
