@@ -80,6 +80,28 @@ def profiled_walk_traffic(args):
     return float(m.group(1)) if m else None
 
 
+def build_roofline(phase, calls, n, key_bits, passes=None):
+    """Achieved HBM rate of the build phases (algorithmic bytes per particle
+    over the phase's event time) against the 8 TB/s peak.  Bytes per
+    particle: kick+drift+bbox 80 (read pos/vel/acc, write pos/vel); Morton 16
+    + key; sort: per pass read and write key + index (the first pass reads no
+    index) + 32 for the last pass's gather of pos_sorted; Karras: key 1x +
+    node 12 + two upward records 32 + arrival counter 4; COM: pos_sorted 16 +
+    two upward records 32 + node com/size 20."""
+    kb = 4 if key_bits == 30 else 8
+    passes = passes or (4 if key_bits == 30 else 8)
+    per = {"bbox": 80.0, "morton": 16.0 + kb,
+           "sort": passes * (2 * kb + 8) - 4 + 32,
+           "build": kb + 12 + 32 + 4, "com": 16 + 32 + 20}
+    out = {}
+    for k, b in per.items():
+        ms = phase[["bbox", "morton", "sort", "build", "com"].index(k)] / max(calls, 1)
+        gbs = b * n / (ms * 1e-3) / 1e9 if ms > 0 else None
+        out[k] = {"bytes_per_particle": b, "ms": ms, "achieved_GBps": gbs,
+                  "frac_of_8TBps": None if gbs is None else gbs / HBM_PEAK_GBS}
+    return out
+
+
 def naive_main(args, sim, eps):
     """All-pairs leapfrog steps; the step time is dominated by the all-pairs
     kernel (the fused kick/drift add ~48 B/particle of HBM traffic)."""
@@ -210,6 +232,7 @@ def ensemble_main(args):
         "autotune": tuned,
         "batched_forces_bitwise_equal_serial": not mismatched,
         "phase_ms": {k: phase[i] / max(calls, 1) for i, k in enumerate(names)},
+        "build_roofline_hbm": build_roofline(phase, calls, n, 63, passes=5),
         "roofline": {"bound": "valu", "kernel": "walk_items",
                      "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
@@ -441,6 +464,7 @@ def bh_main(args, pos, vel, eps, dt, what, metric, data, energy_theta=None, cpu=
                    "n": n, "theta": args.theta, "key_bits": args.key_bits,
                    "traverse": args.traverse, "group_width": width},
         "phase_ms": {k: phase[i] / max(calls, 1) for i, k in enumerate(names)},
+        "build_roofline_hbm": build_roofline(phase, calls, n, args.key_bits),
         "roofline": {"bound": "valu", "kernel": "walk",
                      "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": profiled_walk_traffic(args),

@@ -509,3 +509,25 @@ def test_sort_big_tiles(bits):
     order = np.argsort(keys, kind="stable")
     assert np.array_equal(perm.astype(np.int64), order)
     assert np.array_equal(ks, keys[order])
+
+
+@pytest.mark.parametrize("bits", [30, 63])
+def test_far_outlier(bits):
+    """One particle 1e5 away stretches the bounding box so the rest share a
+    few Morton cells (long runs of equal keys, index tie-breaks): the tree is
+    still the oracle's bit for bit and the forces stay within the BH error."""
+    pos, _ = ic.two_disk_encounter(20000, seed=61, eps=0.05)
+    pos = np.concatenate([pos, np.array([[1e5, -3e4, 2e4, pos[0, 3]]], np.float32)])
+    with engine.Simulation(pos, eps=0.05, theta=0.5, force="bh", traverse="warp",
+                           group_width=64, key_bits=bits) as sim:
+        a = sim.compute_forces()
+        t = sim.tree()
+        idx = np.random.default_rng(1).choice(len(pos), 256, replace=False).astype(np.int32)
+        err = sim.force_error(idx, a)
+    ref = orc.Tree(pos, key_bits=bits)
+    assert np.array_equal(t["keys"], ref.keys_sorted)
+    assert np.array_equal(t["children"], ref.children)
+    assert np.array_equal(t["com4"], ref.com4)
+    print(f"outlier bits={bits}: distinct keys {len(np.unique(t['keys']))}, median err "
+          f"{np.median(err):.2e}", flush=True)
+    assert np.median(err) < 0.01 and np.all(np.isfinite(a))
