@@ -8,17 +8,18 @@
 // d2 = |com - target|^2 in the canonical FP32 order, strict, so theta = 0
 // never accepts.  Children are pushed left then right (right popped first).
 //
-//   walk_thread_kernel   one target per lane, private stack (traverse_kernel).
-//   walk_wave_kernel     group width 64 = one wave64: the whole wave shares
-//                        one path; a node opens if any live lane votes open
-//                        (ballot); the stack lives in two VGPRs, one entry
-//                        per lane (lane-select write / readlane), so the node id is
-//                        wave-uniform and node records come in through scalar
-//                        loads.
-//   walk_group_kernel<W> group widths 16 and 32: several groups per wave,
-//                        each with its own LDS stack and a ballot restricted
-//                        to its lanes; W = 32 is the reference's warp walk
-//                        (traverse_warp_kernel) visit for visit.
+//   walk_thread_kernel    one target per lane, private stack (traverse_kernel).
+//   walk_wave_kernel      the group walk of traverse_warp_kernel at group width
+//                         64, 32 or 16, one group per wave64: a node opens if
+//                         any live lane of the group opens it; node-parallel
+//                         batches (up to 64 nodes, one per lane) settle the
+//                         votes, and an LDS interaction list is evaluated by
+//                         all lanes (walk_group64 below).  Same visited /
+//                         opened / accepted sets, hence the same counts, as the
+//                         one-node-at-a-time warp vote.
+//   walk_items_kernel     the same walk over ensemble work items pulled from a
+//                         device queue (persistent).
+//   walk_potential_kernel the same walk accumulating the potential (tree energy).
 // COUNT variants also record interactions and opening tests (per target for
 // the thread walk, per group otherwise).
 #include "nb_device.hpp"
@@ -509,20 +510,27 @@ __global__ __launch_bounds__(kBlock) void tree_energy_final_kernel(
   }
 }
 
+// One group of `gs` (64, 32 or 16) consecutive targets per wave.  A group
+// narrower than the wave runs the same node-parallel walk with the surplus
+// lanes dead (they repeat the group's last target and vote with it): the
+// visited / opened / accepted sets are exactly those of the gs-wide vote of
+// traverse_warp_kernel, node batches still use all 64 lanes, and only the
+// interaction-list evaluation idles the surplus lanes.
 template <bool COUNT, int MODE>
 __global__ __launch_bounds__(kBlock) void walk_wave_kernel(
     const float4* __restrict__ pos_sorted, const unsigned* __restrict__ perm, int n,
     const Node* __restrict__ nodes, float theta2, float eps2, float G, float4* __restrict__ acc,
     float4* __restrict__ vel, float half_dt, int* __restrict__ counts, int* __restrict__ opens,
-    int* overflow, int* error) {
+    int* overflow, int* error, int gs) {
   constexpr int WAVES = kBlock / 64;
   __shared__ int s_stack[WAVES][WalkMode<MODE>::stack];
   __shared__ float4 s_list[WAVES][kWaveList];
   __shared__ half2_t s_l16[WAVES][WalkMode<MODE>::l16];
   const int w = threadIdx.x >> 6;
-  const int gbase = xcd_block() * blockDim.x + (threadIdx.x & ~63);
+  const int group = xcd_block() * WAVES + w;
+  const int gbase = group * gs;
   if (gbase >= n) return;  // wave-uniform
-  walk_group64<COUNT, MODE>(gbase, min(64, n - gbase), 0, gbase >> 6, pos_sorted, perm, n, nodes,
+  walk_group64<COUNT, MODE>(gbase, min(gs, n - gbase), 0, group, pos_sorted, perm, n, nodes,
                             theta2, eps2, G, acc, vel, half_dt, counts, opens, overflow, error,
                             s_stack[w], s_list[w], s_l16[w]);
 }
@@ -556,74 +564,6 @@ __global__ __launch_bounds__(kBlock) void walk_items_kernel(
   }
 }
 
-template <int W, bool COUNT>
-__global__ __launch_bounds__(kBlock) void walk_group_kernel(
-    const float4* __restrict__ pos_sorted, const unsigned* __restrict__ perm, int n,
-    const Node* __restrict__ nodes, float theta2, float eps2, float G, float4* __restrict__ acc,
-    float4* __restrict__ vel, float half_dt, int* __restrict__ counts, int* __restrict__ opens,
-    int* overflow, int* error) {
-  constexpr int NG = kBlock / W;
-  __shared__ int stk[NG][kStackDepth];
-  const int t = xcd_block() * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  const int g = threadIdx.x / W;
-  const int glane = threadIdx.x % W;
-  const int gbase = t - glane;
-  if (gbase >= n) return;  // group-uniform
-  const bool live = t < n;
-  const int nm1 = n - 1;
-  const unsigned long long gmask = ((1ull << W) - 1ull) << (lane - glane);
-  const float4 b = pos_sorted[live ? t : nm1];
-  float ax = 0.f, ay = 0.f, az = 0.f;
-  if (glane == 0) stk[g][0] = 0;
-  int sp = 1, ni = 0, no = 0;
-  const unsigned nodes_total = 2u * (unsigned)n - 1u;
-  unsigned pops = 0;
-  while (sp > 0) {
-    --sp;
-    const int id = stk[g][sp];
-    if ((unsigned)id >= nodes_total || ++pops > nodes_total) {
-      if (glane == 0) *error = 1;
-      break;
-    }
-    float dx, dy, dz, d2, m;
-    if (id >= nm1) {
-      const float4 src = pos_sorted[id - nm1];
-      dx = src.x - b.x; dy = src.y - b.y; dz = src.z - b.z;
-      d2 = dist2(dx, dy, dz);
-      m = src.w;
-    } else {
-      const float4 c = nodes[id].com;
-      const float s2 = nodes[id].size2;
-      dx = c.x - b.x; dy = c.y - b.y; dz = c.z - b.z;
-      d2 = dist2(dx, dy, dz);
-      if (COUNT) ++no;
-      const bool open = live && !(s2 < theta2 * d2);
-      if (__ballot(open) & gmask) {
-        if (sp + 2 > kStackDepth) {
-          if (glane == 0) *overflow = 1;
-          break;
-        }
-        if (glane == 0) {
-          int2 ch = *reinterpret_cast<const int2*>(&nodes[id].left);
-          stk[g][sp] = ch.x;
-          stk[g][sp + 1] = ch.y;
-        }
-        sp += 2;
-        continue;
-      }
-      m = c.w;
-    }
-    accumulate(dx, dy, dz, d2, m, eps2, ax, ay, az);
-    if (COUNT) ++ni;
-  }
-  if (COUNT && glane == 0) {
-    counts[gbase / W] = ni;
-    opens[gbase / W] = no;
-  }
-  if (live) finish(t, n, ax, ay, az, G, perm, acc, vel, half_dt);
-}
-
 template <bool COUNT>
 void walk_dispatch(hipStream_t s, int traverse, int width, const WalkArgs& a) {
   const int g = (a.n + kBlock - 1) / kBlock;
@@ -632,17 +572,15 @@ void walk_dispatch(hipStream_t s, int traverse, int width, const WalkArgs& a) {
       a.opens, a.overflow, a.error
   if (traverse == 0) {
     walk_thread_kernel<COUNT><<<g, kBlock, 0, s>>>(NB_WALK_ARGS);
-  } else if (width == 64) {
+  } else if (width == 64 || width == 32 || width == 16) {
+    const int groups = (a.n + width - 1) / width;
+    const int gw = (groups + kBlock / 64 - 1) / (kBlock / 64);
     if (a.half_far && !COUNT)
-      walk_wave_kernel<COUNT, 2><<<g, kBlock, 0, s>>>(NB_WALK_ARGS);
+      walk_wave_kernel<COUNT, 2><<<gw, kBlock, 0, s>>>(NB_WALK_ARGS, width);
     else if (a.compensated && !COUNT)
-      walk_wave_kernel<COUNT, 1><<<g, kBlock, 0, s>>>(NB_WALK_ARGS);
+      walk_wave_kernel<COUNT, 1><<<gw, kBlock, 0, s>>>(NB_WALK_ARGS, width);
     else
-      walk_wave_kernel<COUNT, 0><<<g, kBlock, 0, s>>>(NB_WALK_ARGS);
-  } else if (width == 32) {
-    walk_group_kernel<32, COUNT><<<g, kBlock, 0, s>>>(NB_WALK_ARGS);
-  } else if (width == 16) {
-    walk_group_kernel<16, COUNT><<<g, kBlock, 0, s>>>(NB_WALK_ARGS);
+      walk_wave_kernel<COUNT, 0><<<gw, kBlock, 0, s>>>(NB_WALK_ARGS, width);
   } else {
     throw Error(kErrInvalid, "group width must be 16, 32 or 64");
   }
