@@ -149,8 +149,10 @@ class Engine {
     const bool bh = p.force == 1;
     CallEvents ce{};
     const bool timed = timing_on;
-    if (timed) begin_call(ce, bh && phase_timing);
+    // scratch allocation (first call, growth) happens before the timed region:
+    // the force timing covers device work only
     prepare();
+    if (timed) begin_call(ce, bh && phase_timing);
     if (nsys > 0) {
       ens_force_pass(step, ce);
     } else if (bh && n >= 2) {
