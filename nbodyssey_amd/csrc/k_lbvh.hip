@@ -93,14 +93,14 @@ __global__ __launch_bounds__(kBlock) void morton_hist_kernel(const float4* __res
       qp->box_lo[3] = qp->box_hi[3] = 0.f;
     }
   }
-  __syncthreads();
+  lds_barrier();  // s_q; the qp stores are for later launches
   const double lx = s_q[0], ly = s_q[1], lz = s_q[2], inv = s_q[3];
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     K k = make_key<K>(pos[i], lx, ly, lz, inv);
     keys[i] = k;
     for (int p = 0; p < passes; ++p) atomicAdd(&h[p][(unsigned)(k >> (8 * p)) & 255u], 1u);
   }
-  __syncthreads();
+  lds_barrier();  // the histogram; the key stores are for later launches
   for (int p = 0; p < passes; ++p) {
     unsigned c = h[p][threadIdx.x];
     if (c) atomic_add_relaxed(&ss->hist[p][threadIdx.x], c);
@@ -224,7 +224,7 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
     store_agent(row + (size_t)tile * 256 + tid, tag | kFlagInc | (excl + total));
   }
   s_goff[tid] = bucket + excl;
-  __syncthreads();
+  lds_barrier();  // s_goff; the published prefix needs no wait here
 
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j) {
@@ -249,45 +249,76 @@ __device__ __forceinline__ int clz_key(K x) {
   return __clzll((long long)(uint64_t)x);
 }
 
-// delta of forces_bh.cu:234-240 on a 32- or 64-bit key word.
+// Sorted keys around a workgroup's nodes, staged in LDS: kKarrasWin keys on
+// each side of the kBlock nodes.  A node's searches probe at most about twice
+// its range length away, so every node whose range is shorter than the window
+// runs its searches at LDS latency; the few long-range nodes near the top of
+// the tree fall back to global loads for their far probes.
+constexpr int kKarrasWin = 256;
+constexpr int kKarrasStage = kBlock + 2 * kKarrasWin;
+
 template <typename K>
-__device__ __forceinline__ int delta(const K* __restrict__ keys, int n, int i, K ki, int j) {
+struct KeyWindow {
+  const K* __restrict__ keys;
+  const K* s;
+  int wlo;
+  __device__ __forceinline__ K operator()(int j) const {
+    const unsigned k = (unsigned)(j - wlo);
+    return k < (unsigned)kKarrasStage ? s[k] : keys[j];
+  }
+};
+
+// delta of forces_bh.cu:234-240 through a key window; -1 outside [0, n).
+template <typename K>
+__device__ __forceinline__ int delta_w(const KeyWindow<K>& key, int n, int i, K ki, int j) {
   if (j < 0 || j >= n) return -1;
-  K x = ki ^ keys[j];
+  K x = ki ^ key(j);
   if (x) return clz_key<K>(x);
   return (int)(8 * sizeof(K)) + __clz(i ^ j);
 }
 
+// Karras (2012) construction, forces_bh.cu:242-279: direction from the
+// neighbours' common prefixes, range end by doubling then bisection, split by
+// bisection on the common-prefix length.
 template <typename K>
 __global__ __launch_bounds__(kBlock) void karras_kernel(const K* __restrict__ keys, int n,
                                                         Node* __restrict__ nodes,
                                                         int4* __restrict__ up,
                                                         unsigned* __restrict__ visit,
                                                         SortState* ss, int passes) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ K s_key[kKarrasStage];
+  const int base = blockIdx.x * kBlock;
+  const int wlo = base - kKarrasWin;
+  for (int k = threadIdx.x; k < kKarrasStage; k += kBlock) {
+    const int j = wlo + k;
+    if (j >= 0 && j < n) s_key[k] = keys[j];
+  }
   if (ss && blockIdx.x == 0) {
     // the sort of this step is complete: reset its state for the next one
     for (int p = 0; p < passes; ++p) ss->hist[p][threadIdx.x] = 0;
     if (threadIdx.x < kMaxPasses) ss->ticket[threadIdx.x] = 0;
     if (threadIdx.x == 0) ss->epoch = ss->epoch + 1;
   }
+  __syncthreads();
+  const KeyWindow<K> key{keys, s_key, wlo};
+  const int i = base + threadIdx.x;
   if (i >= n - 1) return;
-  const K ki = keys[i];
-  int d = (delta(keys, n, i, ki, i + 1) - delta(keys, n, i, ki, i - 1)) >= 0 ? 1 : -1;
-  int dmin = delta(keys, n, i, ki, i - d);
+  const K ki = key(i);
+  int d = (delta_w(key, n, i, ki, i + 1) - delta_w(key, n, i, ki, i - 1)) >= 0 ? 1 : -1;
+  int dmin = delta_w(key, n, i, ki, i - d);
   int lmax = 2;
-  while (delta(keys, n, i, ki, i + lmax * d) > dmin) lmax <<= 1;
+  while (delta_w(key, n, i, ki, i + lmax * d) > dmin) lmax <<= 1;
   int l = 0;
   for (int t = lmax >> 1; t > 0; t >>= 1)
-    if (delta(keys, n, i, ki, i + (l + t) * d) > dmin) l += t;
+    if (delta_w(key, n, i, ki, i + (l + t) * d) > dmin) l += t;
   int j = i + l * d;
   int first = min(i, j), last = max(i, j);
-  const K kf = keys[first];
-  int common = delta(keys, n, first, kf, last);
+  const K kf = key(first);
+  int common = delta_w(key, n, first, kf, last);
   int split = first, step = last - first;
   do {
     step = (step + 1) >> 1;
-    if (split + step < last && delta(keys, n, first, kf, split + step) > common) split += step;
+    if (split + step < last && delta_w(key, n, first, kf, split + step) > common) split += step;
   } while (step > 1);
   int left = (split == first) ? (n - 1 + split) : split;
   int right = (split + 1 == last) ? (n - 1 + split + 1) : split + 1;
@@ -305,16 +336,18 @@ struct NodeData {
   float4 com, lo, hi;
 };
 
-__device__ __forceinline__ NodeData leaf_data(const float4* __restrict__ pos_sorted, int leaf) {
-  const float4 p = pos_sorted[leaf];
-  return NodeData{p, p, p};
-}
-
+// A leaf's data is its position; an internal node's is its COM and AABB.  The
+// three loads are issued unconditionally from selected addresses (a leaf reads
+// its position three times), so a wave with both kinds of lanes waits for one
+// round trip, not two.
 __device__ __forceinline__ NodeData load_data(const float4* __restrict__ pos_sorted, int nm1,
                                               const Node* nodes, const float4* box_lo,
                                               const float4* box_hi, int id) {
-  if (id >= nm1) return leaf_data(pos_sorted, id - nm1);
-  return NodeData{nodes[id].com, box_lo[id], box_hi[id]};
+  const bool leaf = id >= nm1;
+  const float4* pc = leaf ? pos_sorted + (id - nm1) : &nodes[id].com;
+  const float4* pl = leaf ? pc : box_lo + id;
+  const float4* ph = leaf ? pc : box_hi + id;
+  return NodeData{*pc, *pl, *ph};
 }
 
 // Combine two children (forces_bh.cu:313-324): FP64 mass-weighted centre of
@@ -363,21 +396,63 @@ __device__ __forceinline__ void seg_push(int* s_list, unsigned* s_cnt, int v, in
 // Pass 1: workgroup b owns the sorted leaves [b*L, b*L + L), one per thread.
 // Internal node i covers the leaves between i and its `other` end, so a node
 // whose range lies inside the workgroup's leaves has its id in [lo, hi] and is
-// reached only by this workgroup.  The workgroup preloads the upward records
-// of its leaves and of internal nodes lo..hi into LDS (coalesced), keeps the
-// arrival counters and the finished nodes' COM/AABB in LDS, and so climbs
-// without dependent global loads: global memory sees the leaf positions, the
-// final node stores and the frontier.  A climb that reaches a node spanning
-// workgroups stops and records the finished child (with its AABB, which only
-// the later passes need) in the workgroup's frontier segment.
+// reached only by this workgroup.  The workgroup stages its leaf positions and
+// the upward records of internal nodes lo..hi in LDS (coalesced) and keeps the
+// arrival counters and the finished nodes' COM/AABB in LDS, so no global load
+// is waited for after staging; node stores stream out unawaited.
+//
+// The nodes are combined in rounds, not by one climbing thread per leaf: a
+// per-leaf climb keeps a wave alive while any of its lanes climbs, and on a
+// two-disk tree of 2^20 leaves that left 90% of the lanes idle.  Round r holds
+// the parents that became owned in round r-1 (the second arrival at a
+// parent's counter owns it), packed densely into threads 0..count-1 as
+// (child, sibling, parent).  A round is a chain of LDS round trips (about
+// 85 ns per read, 125 ns per atomic and per barrier on MI355X), so a round
+// reads an entry, then both children's data with the parent's upward record
+// at once, and one barrier separates rounds (three rotating counters).  A
+// thread that owns the next parent right away combines it too (kClimb more
+// combines per round), which halves the rounds of a 1024-leaf block (13 ->
+// 7) without bringing back the idle lanes.  The combine is symmetric in the
+// two children, so the result does not depend on which child arrives second.
+// A child whose parent spans workgroups is a frontier entry (with its AABB,
+// which only the later passes need).
+//
+// Local slots: internal node lo+k is k, leaf lo+k is kComLeaves+k.  An upward
+// record in LDS is (parent slot, sibling slot), with parent = kUpRoot at the
+// root and kUpOut when the parent's range leaves the workgroup.
+constexpr int kUpRoot = -1;
+constexpr int kUpOut = -2;
+static_assert(kComLeaves == 1024, "list entries pack 11-bit child slots and 10-bit parents");
+// Combines a thread may chain within one round before it hands the next owned
+// parent to the following round.
+constexpr int kClimb = 1;
+
+__device__ __forceinline__ int2 local_up(int4 u, int lo, int hi, int nm1, int total, int* error) {
+  if (u.x < 0) return make_int2(kUpRoot, 0);
+  if (u.x >= nm1 || (unsigned)u.y >= (unsigned)total) {
+    *error = 1;
+    return make_int2(kUpRoot, 0);
+  }
+  if (min(u.x, u.z) < lo || max(u.x, u.z) > hi) return make_int2(kUpOut, 0);
+  // a local parent's children lie inside its range, so the sibling is local
+  const int sib = u.y >= nm1 ? u.y - nm1 : u.y;
+  if (sib < lo || sib > hi) {
+    *error = 1;
+    return make_int2(kUpRoot, 0);
+  }
+  return make_int2(u.x - lo, u.y >= nm1 ? kComLeaves + sib - lo : sib - lo);
+}
+
 __global__ __launch_bounds__(kComThreads) void com_local_kernel(
     const float4* __restrict__ pos_sorted, int n, Node* nodes, const int4* __restrict__ up,
     float4* box_lo, float4* box_hi, int* seg1, unsigned* cnt1, int* error) {
-  __shared__ unsigned s_visit[kComLeaves];
-  __shared__ int3 s_up_leaf[kComLeaves];
-  __shared__ int3 s_up_node[kComLeaves];
+  __shared__ float4 s_leaf[kComLeaves];
   __shared__ float4 s_com[kComLeaves];
   __shared__ float3 s_lo[kComLeaves], s_hi[kComLeaves];
+  __shared__ int2 s_up[kComLeaves];              // internal slots
+  __shared__ unsigned s_visit[kComLeaves / 16];  // 2-bit arrival counters
+  __shared__ unsigned s_list[2][kComLeaves];     // child | sibling << 11 | parent << 22
+  __shared__ unsigned s_cnt[3];
   __shared__ int s_front[kSeg];
   __shared__ unsigned s_nfront;
   const int nm1 = n - 1;
@@ -385,74 +460,156 @@ __global__ __launch_bounds__(kComThreads) void com_local_kernel(
   const int lo = blockIdx.x * kComLeaves;
   const int hi = min(n - 1, lo + kComLeaves - 1);
   const int t = threadIdx.x;
-  s_visit[t] = 0;
+  if (t < kComLeaves / 16) s_visit[t] = 0;
+  if (t < 3) s_cnt[t] = 0;
   if (t == 0) s_nfront = 0;
+  int2 leaf_up = make_int2(kUpRoot, 0);
   if (lo + t <= hi) {
+    const float4 p = pos_sorted[lo + t];
     const int4 ul = up[nm1 + lo + t];
-    s_up_leaf[t] = make_int3(ul.x, ul.y, ul.z);
-    if (lo + t < nm1) {
-      const int4 un = up[lo + t];
-      s_up_node[t] = make_int3(un.x, un.y, un.z);
-    }
+    int4 un = make_int4(kUpRoot, 0, 0, 0);
+    if (lo + t < nm1) un = up[lo + t];
+    s_leaf[t] = p;
+    leaf_up = local_up(ul, lo, hi, nm1, total, error);
+    if (lo + t < nm1) s_up[t] = local_up(un, lo, hi, nm1, total, error);
   }
   __syncthreads();
-  const int leaf = lo + t;
-  if (leaf <= hi) {
-    int child = nm1 + leaf;
-    NodeData cur = leaf_data(pos_sorted, leaf);
-    int3 u = s_up_leaf[t];
-    while (true) {
-      const int node = u.x;
-      if (node < 0) break;  // reached the root
-      if (node >= nm1 || (unsigned)u.y >= (unsigned)total) {
-        *error = 1;
-        break;
-      }
-      if (min(node, u.z) < lo || max(node, u.z) > hi) {
-        // the finished child is a frontier entry: later passes need its AABB
-        if (child < nm1) {
-          box_lo[child] = cur.lo;
-          box_hi[child] = cur.hi;
-        }
-        seg_push(s_front, &s_nfront, child, error);
-        break;
-      }
-      const int slot = node - lo;
-      const unsigned prev = __hip_atomic_fetch_add(&s_visit[slot], 1u, __ATOMIC_ACQ_REL,
-                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
-      if (prev != 1u) {  // first arrival stops; a third would mean a malformed tree
-        if (prev > 1u) *error = 1;
-        break;
-      }
-      // the sibling is a local leaf (global) or a local internal node (LDS)
-      NodeData sib;
-      if (u.y >= nm1) {
-        sib = leaf_data(pos_sorted, u.y - nm1);
+
+  // A child's data is read from both layouts at once and then selected.  The
+  // empty asm statements consume every value read, so the compiler can
+  // neither sink the reads into branches nor wait between them.
+  struct Raw {
+    float4 q, m;
+    float3 l, h;
+  };
+  auto raw = [&](int c) {
+    const int k = c & (kComLeaves - 1);
+    return Raw{s_leaf[k], s_com[k], s_lo[k], s_hi[k]};
+  };
+  auto pin = [](Raw& a) {
+    asm volatile("" : "+v"(a.q.x), "+v"(a.q.y), "+v"(a.q.z), "+v"(a.q.w), "+v"(a.m.x),
+                 "+v"(a.m.y), "+v"(a.m.z), "+v"(a.m.w), "+v"(a.l.x), "+v"(a.l.y), "+v"(a.l.z),
+                 "+v"(a.h.x), "+v"(a.h.y), "+v"(a.h.z));
+  };
+  auto select = [](const Raw& a, int c) {
+    const bool leaf = c >= kComLeaves;
+    NodeData d;
+    d.com = leaf ? a.q : a.m;
+    d.lo = leaf ? a.q : make_float4(a.l.x, a.l.y, a.l.z, 0.f);
+    d.hi = leaf ? a.q : make_float4(a.h.x, a.h.y, a.h.z, 0.f);
+    return d;
+  };
+  // a finished internal node: LDS for its parent, global memory for the walk
+  auto put = [&](int slot, const NodeData& d) {
+    s_com[slot] = d.com;
+    s_lo[slot] = make_float3(d.lo.x, d.lo.y, d.lo.z);
+    s_hi[slot] = make_float3(d.hi.x, d.hi.y, d.hi.z);
+    const float sz = fmaxf(d.hi.x - d.lo.x, fmaxf(d.hi.y - d.lo.y, d.hi.z - d.lo.z));
+    nodes[lo + slot].com = d.com;
+    nodes[lo + slot].size2 = sz * sz;
+  };
+  // A finished child (slot c, data d, upward record u) reports to its parent;
+  // true when this thread arrived second and so owns the parent.  The arrival
+  // orders the child's LDS data (release) before a sibling's read (acquire);
+  // both are LDS-only, so no global store is waited for.
+  auto report = [&](int c, const NodeData& d, int2 u) {
+    if (u.x == kUpRoot) return false;
+    if (u.x == kUpOut) {  // the parent spans workgroups: a frontier entry
+      if (c < kComLeaves) {
+        box_lo[lo + c] = d.lo;
+        box_hi[lo + c] = d.hi;
+        seg_push(s_front, &s_nfront, lo + c, error);
       } else {
-        const int ss = u.y - lo;
-        sib.com = s_com[ss];
-        sib.lo = make_float4(s_lo[ss].x, s_lo[ss].y, s_lo[ss].z, 0.f);
-        sib.hi = make_float4(s_hi[ss].x, s_hi[ss].y, s_hi[ss].z, 0.f);
+        seg_push(s_front, &s_nfront, nm1 + lo + c - kComLeaves, error);
       }
-      cur = combine(cur, sib);
-      s_com[slot] = cur.com;
-      s_lo[slot] = make_float3(cur.lo.x, cur.lo.y, cur.lo.z);
-      s_hi[slot] = make_float3(cur.hi.x, cur.hi.y, cur.hi.z);
-      const float sz = fmaxf(cur.hi.x - cur.lo.x, fmaxf(cur.hi.y - cur.lo.y, cur.hi.z - cur.lo.z));
-      nodes[node].com = cur.com;
-      nodes[node].size2 = sz * sz;
-      child = node;
-      u = s_up_node[slot];
+      return false;
+    }
+    const unsigned shift = 2u * (unsigned)(u.x & 15);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    const unsigned old = __hip_atomic_fetch_add(&s_visit[u.x >> 4], 1u << shift,
+                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    const unsigned prev = (old >> shift) & 3u;
+    if (prev > 1u) *error = 1;
+    return prev == 1u;
+  };
+  // Climb from finished child c while this thread owns the parent, for at
+  // most kClimb combines; an owned parent beyond that is appended, as
+  // (child, sibling, parent), to list `list` under counter `ctr`.
+  auto climb = [&](int c, NodeData d, int2 u, int ctr, int list) {
+    for (int level = 0;; ++level) {
+      if (!report(c, d, u)) return;
+      if (level == kClimb) {
+        const unsigned k = __hip_atomic_fetch_add(&s_cnt[ctr], 1u, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+        const unsigned e = (unsigned)c | ((unsigned)u.y << 11) | ((unsigned)u.x << 22);
+        if (k < (unsigned)kComLeaves) s_list[list][k] = e;
+        else *error = 1;  // more owned parents than nodes: a malformed tree
+        return;
+      }
+      Raw rq = raw(u.y);
+      int2 un = s_up[u.x];
+      pin(rq);
+      asm volatile("" : "+v"(un.x), "+v"(un.y));
+      d = combine(d, select(rq, u.y));
+      c = u.x;
+      u = un;
+      put(c, d);
+    }
+  };
+
+  // round 0: the leaves (list 1 / counter 1)
+  if (lo + t <= hi) {
+    const float4 q = s_leaf[t];
+    climb(kComLeaves + t, NodeData{q, q, q}, leaf_up, 1, 1);
+  }
+  // Round r reads counter r % 3 and list r & 1, appends to counter (r+1) % 3
+  // and list (r+1) & 1, and clears counter (r+2) % 3, which every thread read
+  // before this round's barrier.  A subtree of L leaves is at most L - 1
+  // levels high, so a valid tree ends within kComLeaves rounds; the bound only
+  // keeps a malformed one finite.
+  for (int r = 1;; ++r) {
+    lds_barrier();
+    unsigned cnt = s_cnt[r % 3];
+    unsigned e = s_list[r & 1][t];  // read with the count, used only below it
+    asm volatile("" : "+v"(cnt), "+v"(e));
+    if (cnt == 0) break;
+    if (r > kComLeaves) {
+      if (t == 0) *error = 1;
+      break;
+    }
+    if (t == 0) s_cnt[(r + 2) % 3] = 0;
+    if (t < (int)cnt) {
+      const int c = (int)(e & 0x7ffu), sb = (int)((e >> 11) & 0x7ffu), slot = (int)(e >> 22);
+      Raw rc = raw(c), rs = raw(sb);
+      int2 u = s_up[slot];
+      pin(rc);
+      pin(rs);
+      asm volatile("" : "+v"(u.x), "+v"(u.y));
+      const NodeData d = combine(select(rc, c), select(rs, sb));
+      put(slot, d);
+      climb(slot, d, u, (r + 1) % 3, (r + 1) & 1);
     }
   }
-  __syncthreads();
   const unsigned cnt = min(s_nfront, (unsigned)kSeg);
   if (t < (int)cnt) seg1[blockIdx.x * kSeg + t] = s_front[t];
   if (t == 0) cnt1[blockIdx.x] = cnt;
 }
 
-// Climb through global node data with arrival counters in LDS, shared by
-// passes 2 and 3.  `counter(node)` returns the previous arrival count.
+// Where a climb goes from an upward record: 0 stop at the root, 1 malformed
+// record, 2 the parent leaves [lo, hi] (frontier), 3 climb to the parent.
+// Every field of the record is consumed here, before any branch, so the
+// record is fetched as one 16-byte load.
+__device__ __forceinline__ int climb_status(int4 u, int nm1, int total, int lo, int hi) {
+  const bool bad = u.x >= nm1 || (unsigned)u.y >= (unsigned)total;
+  const bool out = min(u.x, u.z) < lo || max(u.x, u.z) > hi;
+  return u.x < 0 ? 0 : bad ? 1 : out ? 2 : 3;
+}
+
+// Climb through global node data, shared by passes 2 and 3.  `counter(node)`
+// returns the previous arrival count.  The parent's upward record was written
+// by an earlier launch, so it is fetched together with the sibling's data:
+// one dependent global round trip per level.
 template <typename Counter>
 __device__ __forceinline__ void climb_global(const float4* __restrict__ pos_sorted, int n,
                                              Node* nodes, const int4* __restrict__ up,
@@ -462,26 +619,29 @@ __device__ __forceinline__ void climb_global(const float4* __restrict__ pos_sort
   const int nm1 = n - 1;
   const int total = 2 * n - 1;
   NodeData cur = load_data(pos_sorted, nm1, nodes, box_lo, box_hi, child);
+  int4 u = up[child];
   while (true) {
-    const int4 u = up[child];
-    const int node = u.x;
-    if (node < 0) break;
-    if (node >= nm1 || (unsigned)u.y >= (unsigned)total) {
+    const int st = climb_status(u, nm1, total, lo, hi);
+    if (st == 0) break;
+    if (st == 1) {
       *error = 1;
       break;
     }
-    if (min(node, u.z) < lo || max(node, u.z) > hi) {
+    if (st == 2) {
       seg_push(s_list, s_cnt, child, error);
       break;
     }
+    const int node = u.x;
     const unsigned prev = counter(node);
     if (prev != 1u) {
       if (prev > 1u) *error = 1;
       break;
     }
+    const int4 next = up[node];
     cur = combine(cur, load_data(pos_sorted, nm1, nodes, box_lo, box_hi, u.y));
     store_node(nodes, box_lo, box_hi, node, cur);
     child = node;
+    u = next;
   }
 }
 
@@ -510,15 +670,16 @@ __global__ __launch_bounds__(kComThreads) void com_mid_kernel(
                                                 __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
     return (old >> shift) & 3u;
   };
-  // compact the segments' entries so that entry j goes to thread j
+  // compact the segments' entries so that entry j goes to thread j: one wave
+  // loads the kPer counts together and scans them
+  static_assert(kPer == 64, "the segment scan is one wave wide");
   __shared__ unsigned s_off[kPer + 1];
-  if (threadIdx.x == 0) {
-    unsigned acc = 0;
-    for (int b = 0; b < kPer; ++b) {
-      s_off[b] = acc;
-      acc += (b0 + b < nblocks1) ? cnt1[b0 + b] : 0u;
-    }
-    s_off[kPer] = acc;
+  if (threadIdx.x < kPer) {
+    const int b = threadIdx.x;
+    const unsigned c = (b0 + b < nblocks1) ? cnt1[b0 + b] : 0u;
+    const unsigned incl = wave_incl_scan(c);
+    s_off[b + 1] = incl;
+    if (b == 0) s_off[0] = 0;
   }
   __syncthreads();
   for (unsigned j = threadIdx.x; j < s_off[kPer]; j += blockDim.x) {
@@ -547,16 +708,30 @@ __global__ __launch_bounds__(kComThreads) void com_top_kernel(
   if (threadIdx.x == 0) s_dummy_cnt = 0;
   __syncthreads();
   auto counter = [&](int node) { return atomic_add_wg(&visit[node], 1u); };
-  // compact the entries of all superblock segments so entry j goes to thread j
+  // compact the entries of all superblock segments so entry j goes to thread
+  // j: each thread loads kTopPer consecutive counts, then a block-wide scan
+  constexpr int kTopPer = kTopSegs / kComThreads;
+  constexpr int kWaves = kComThreads / 64;
   __shared__ unsigned s_off[kTopSegs + 1];
-  if (threadIdx.x == 0) {
-    unsigned acc = 0;
-    for (int b = 0; b < nblocks2; ++b) {
-      s_off[b] = acc;
-      acc += cnt2[b];
-    }
-    s_off[nblocks2] = acc;
+  __shared__ unsigned s_wave[kWaves];
+  unsigned c[kTopPer];
+  unsigned sum = 0;
+  for (int k = 0; k < kTopPer; ++k) {
+    const int b = threadIdx.x * kTopPer + k;
+    c[k] = b < nblocks2 ? cnt2[b] : 0u;
+    sum += c[k];
   }
+  const unsigned incl = wave_incl_scan(sum);
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 63) s_wave[wave] = incl;
+  __syncthreads();
+  unsigned base = incl - sum;
+  for (int w = 0; w < wave; ++w) base += s_wave[w];
+  for (int k = 0; k < kTopPer; ++k) {
+    s_off[threadIdx.x * kTopPer + k] = base;
+    base += c[k];
+  }
+  if (threadIdx.x == kComThreads - 1) s_off[kTopSegs] = base;
   __syncthreads();
   const unsigned total_entries = s_off[nblocks2];
   for (unsigned j = threadIdx.x; j < total_entries; j += blockDim.x) {

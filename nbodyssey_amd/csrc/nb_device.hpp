@@ -91,6 +91,16 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Workgroup barrier that orders LDS only.  __syncthreads() also makes this
+// wave's global stores visible to the workgroup, which waits for every store
+// in flight (a round trip to L2) at each barrier; a loop that communicates
+// through LDS and only streams results out to global memory does not need it.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Inclusive prefix sum over the 64 lanes of a wave.
 __device__ __forceinline__ unsigned wave_incl_scan(unsigned v) {
   int lane = lane_id();
