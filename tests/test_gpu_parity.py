@@ -98,6 +98,22 @@ def test_tree_pipeline_bit_exact(case, bits):
     assert np.array_equal(t["size2"], ref.size2)
 
 
+@pytest.mark.parametrize("n", [200003, 1200000])
+def test_com_passes_bit_exact(n):
+    """COM is bit-exact to the oracle through all three passes: 1024-leaf
+    blocks, 65,536-leaf superblocks and the top (19 superblocks at 1.2M)."""
+    pos = ic.two_disk_encounter(n, seed=9, eps=0.02)[0]
+    print(f"com passes n={n}", flush=True)
+    with engine.Simulation(pos, eps=0.02, theta=0.5, force="bh", traverse="warp",
+                           group_width=64, key_bits=30) as sim:
+        sim.compute_forces()
+        t = sim.tree()
+    ref = orc.Tree(pos, key_bits=30)
+    assert np.array_equal(t["children"], ref.children)
+    assert np.array_equal(t["com4"], ref.com4)
+    assert np.array_equal(t["size2"], ref.size2)
+
+
 @pytest.mark.parametrize("theta", [0.0, 0.5, 0.8])
 @pytest.mark.parametrize("walk", [("thread", 1), ("warp", 16), ("warp", 32), ("warp", 64)])
 @pytest.mark.parametrize("case", ["blobs3024", "disk5000"])
