@@ -196,7 +196,7 @@ def ensemble_main(args):
             opens_sum += to.astype(np.float64).sum()
             idx = rng.choice(m, per_sys, replace=False).astype(np.int32)
             err.append(sim.force_error(idx, a_s))
-            sim.set_graph(True)
+            sim.set_graph(not args.no_graph)
             sim.step(args.serial_warmup)
             sim.synchronize()
             ms_serial += sim.time_steps(args.serial_steps) / args.serial_steps
@@ -223,11 +223,12 @@ def ensemble_main(args):
         "config": {"workload": f"ensemble of {args.systems} two-disk encounters, {n} particles "
                                f"(n_s log-uniform 2048..65536), Barnes-Hut rebuilt every step, "
                                f"(system, 30-bit) keys, theta={theta}, eps={eps}, dt={dt}, "
-                               f"wave64 persistent walk, graph-captured step",
+                               f"wave64 persistent walk, "
+                               f"{'eager' if args.no_graph else 'graph-captured'} step",
                    "systems": args.systems, "n": n, "theta": theta},
         "serial": {"value": serial_value, "ms_per_ensemble_step": ms_serial,
                    "how": "each system alone on the plain path (bh, wave64, 30-bit keys, "
-                          "graph mode), summed"},
+                          + ("eager" if args.no_graph else "graph mode") + "), summed"},
         "batched_over_serial": value / serial_value,
         "autotune": tuned,
         "batched_forces_bitwise_equal_serial": not mismatched,
@@ -339,7 +340,8 @@ def main():
                     help="composite: theta of the tree-potential energy taken before and after "
                          "the timed steps (0 = none)")
     ap.add_argument("--no-graph", action="store_true",
-                    help="ensemble: time eager steps instead of graph replays")
+                    help="ensemble: time eager steps instead of graph replays, batched and "
+                         "serial (for kernel tracing)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

@@ -795,10 +795,12 @@ int sort_passes(int key_bits) { return key_bits == 30 ? 4 : 8; }
 void launch_morton_hist(hipStream_t s, const float4* pos, int n, const float4* partial,
                         int nparts, QuantParams* qp, int key_bits, void* keys, SortState* ss) {
   if (n <= 0) return;
-  // 16 keys per thread: every workgroup first reduces all bbox partials and
-  // ends with its histogram atomics, so fewer, fuller workgroups pay less of
-  // both (1M keys: 512 -> 256 workgroups, 18.1 -> 16.2 us; 128 was slower)
-  int g = grid_for(n, 16, 2048);
+  // Every workgroup first reduces all bbox partials and ends with its
+  // histogram atomics, so at large n fewer, fuller workgroups pay less of both
+  // (1M keys at 16 per thread: 512 -> 256 workgroups, 18.1 -> 16.2 us; 128
+  // was slower); small systems keep 8 per thread for parallelism (at 2K-64K
+  // keys 16 per thread took 10.9 us on average instead of 7.9).
+  int g = grid_for(n, n >= (1 << 19) ? 16 : 8, 2048);
   if (key_bits == 30)
     morton_hist_kernel<uint32_t><<<g, kBlock, 0, s>>>(pos, n, partial, nparts, qp,
                                                       (uint32_t*)keys, ss, 4);
