@@ -8,13 +8,21 @@
 // that parse the reference's output (gpu_bench.sh:31,41-44) work unchanged.
 // Additive flags: --keys 30|63, --group-width 16|32|64, --compensated, --far16,
 // --energy-theta f (energy log from the tree potential at that theta instead
-// of the O(n^2) sum: the cost policy for large n).
+// of the O(n^2) sum: the cost policy for large n), --checkpoint path (write
+// the final state, positions and velocities, in the ic.bin format, so that
+// --ic path resumes the run).
+//
+// Frame files are written by a second host thread while the steps go on: a
+// dump copies the positions to one of two host buffers (the copy the
+// reference makes too) and hands the buffer to the writer, which the next
+// dump waits for before reusing the other buffer.
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <filesystem>
 #include <fstream>
+#include <future>
 #include <string>
 #include <vector>
 
@@ -34,6 +42,7 @@ struct Args {
   int dump_every = 5;
   bool compare = false;
   float energy_theta = 0.f;  // 0: exact O(n^2) energy
+  std::string checkpoint;    // empty: none
   SimParams params{/*dt=*/0.01f, /*epsilon=*/0.05f, /*G=*/1.0f};
 };
 
@@ -105,6 +114,8 @@ Args parse_args(int argc, char** argv) {
       a.params.compensated = true;
     } else if (k == "--far16") {
       a.params.far16 = true;
+    } else if (k == "--checkpoint") {
+      a.checkpoint = next();
     } else if (k == "--energy-theta") {
       a.energy_theta = to_float(k, next());
       if (!(a.energy_theta >= 0.f && a.energy_theta < 0.577f))
@@ -117,7 +128,7 @@ Args parse_args(int argc, char** argv) {
     die("usage: galaxy_sim --ic ic.bin [--steps N] [--dt f] [--eps f] [--dump-every N] "
         "[--out dir] [--force naive|bh] [--theta f] [--traverse thread|warp] "
         "[--compare-forces] [--keys 30|63] [--group-width 16|32|64] [--compensated] "
-        "[--far16] [--energy-theta f]");
+        "[--far16] [--energy-theta f] [--checkpoint path]");
   if (a.dump_every <= 0) die("--dump-every must be positive");
   return a;
 }
@@ -171,7 +182,7 @@ int load_ic(const std::string& path, std::vector<float4>& pos, std::vector<float
   return n;
 }
 
-// main.cu:130-138
+// main.cu:130-138, on the writer thread (unchecked writes, as there).
 void dump_frame(const std::vector<float4>& pos, const std::string& dir, int frame) {
   char name[64];
   std::snprintf(name, sizeof(name), "frame_%05d.bin", frame);
@@ -179,6 +190,23 @@ void dump_frame(const std::vector<float4>& pos, const std::string& dir, int fram
   std::int32_t n = static_cast<std::int32_t>(pos.size());
   f.write(reinterpret_cast<const char*>(&n), sizeof(n));
   f.write(reinterpret_cast<const char*>(pos.data()), n * sizeof(float4));
+}
+
+// The full state in the ic.bin layout (docs/FORMATS.md:7-18): int32 n, then
+// float4 pos[n], then float4 vel[n].
+void write_checkpoint(const ParticleSystem& sys, const std::string& path) {
+  std::vector<float4> pos(sys.n), vel(sys.n);
+  sys.download_positions(pos.data());
+  if (hipMemcpy(vel.data(), sys.d_vel, sys.n * sizeof(float4), hipMemcpyDeviceToHost) !=
+      hipSuccess)
+    die("device copy failed for the checkpoint velocities");
+  if (fs::path(path).has_parent_path()) fs::create_directories(fs::path(path).parent_path());
+  std::ofstream f(path, std::ios::binary);
+  std::int32_t n = sys.n;
+  f.write(reinterpret_cast<const char*>(&n), sizeof(n));
+  f.write(reinterpret_cast<const char*>(pos.data()), n * sizeof(float4));
+  f.write(reinterpret_cast<const char*>(vel.data()), n * sizeof(float4));
+  if (!f) die("cannot write checkpoint: " + path);
 }
 
 }  // namespace
@@ -226,19 +254,28 @@ int main(int argc, char** argv) {
     elog << line;
   };
 
-  std::vector<float4> frame_buf(n);
+  std::vector<float4> frame_buf[2] = {std::vector<float4>(n), std::vector<float4>(n)};
+  std::future<void> writing;  // the frame in flight on the writer thread
+  auto wait_writer = [&]() {
+    if (writing.valid()) writing.get();
+  };
   int frame = 0;
   reset_force_timing();
   compute_forces(sys, d_acc, args.params);  // a(x0) for the first kick
   for (int step = 0; step < args.steps; ++step) {
     leapfrog_step(sys, d_acc, args.params);
     if (step % args.dump_every == 0) {
-      sys.download_positions(frame_buf.data());
-      dump_frame(frame_buf, args.out_dir, frame++);
+      std::vector<float4>& buf = frame_buf[frame & 1];
+      wait_writer();  // the other buffer's frame; this one was written two dumps ago
+      sys.download_positions(buf.data());
+      writing = std::async(std::launch::async, dump_frame, std::cref(buf),
+                           std::cref(args.out_dir), frame++);
       log_energy(step);
     }
   }
   log_energy(args.steps);
+  wait_writer();
+  if (!args.checkpoint.empty()) write_checkpoint(sys, args.checkpoint);
 
   double ms = force_kernel_ms_total();
   long calls = force_kernel_calls();

@@ -346,6 +346,42 @@ def test_galaxy_sim_end_to_end(tmp_path):
     assert mean < 1e-4
 
 
+@pytest.mark.parametrize("force", ["bh", "naive"])
+def test_galaxy_sim_checkpoint_resume(tmp_path, force):
+    """--checkpoint writes the final (pos, vel) in the ic.bin layout; resuming
+    from a 5-step checkpoint reproduces steps 6-10 of a straight 10-step run
+    bit for bit (the force pipeline is deterministic), and every frame the
+    writer thread wrote is the snapshot of its own step."""
+    import subprocess
+    from nbodyssey_amd import GALAXY_SIM, formats
+    pos, vel = ic.generate_ic_disks(particles=3000, seed=4)
+    icp = tmp_path / "ic.bin"
+    formats.write_ic(icp, pos, vel)
+
+    def run(ic_path, steps, tag):
+        out = tmp_path / f"frames_{tag}"
+        ck = tmp_path / f"ck_{tag}.bin"
+        r = subprocess.run([GALAXY_SIM, "--ic", str(ic_path), "--steps", str(steps),
+                            "--dump-every", "1", "--out", str(out), "--energy-log",
+                            str(tmp_path / f"e_{tag}.csv"), "--force", force, "--traverse",
+                            "warp", "--checkpoint", str(ck)], capture_output=True, text=True,
+                           timeout=120)
+        print(tag, r.stdout, r.stderr, flush=True)
+        assert r.returncode == 0
+        frames = [formats.read_frame(out / f"frame_{k:05d}.bin") for k in range(steps)]
+        return frames, formats.read_ic(ck)
+
+    straight, (p10, v10) = run(icp, 10, "straight")
+    _, (p5, v5) = run(icp, 5, "first")
+    resumed, (p10r, v10r) = run(tmp_path / "ck_first.bin", 5, "resumed")
+    assert np.array_equal(p10, straight[9])          # the last frame is the final state
+    assert np.array_equal(p5, straight[4])
+    for k in range(5):
+        assert np.array_equal(resumed[k], straight[5 + k])
+    assert np.array_equal(p10r, p10) and np.array_equal(v10r, v10)
+    assert not np.array_equal(v10, v5)
+
+
 @pytest.mark.parametrize("n", [2, 3, 1000, 65537])
 def test_direct_f64_vs_oracle(n):
     """nb_direct_f64 (the FP64 accuracy reference on the device) equals the
