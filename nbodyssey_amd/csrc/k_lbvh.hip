@@ -744,6 +744,112 @@ __global__ __launch_bounds__(kComThreads) void com_top_kernel(
   }
 }
 
+// Small inputs: one workgroup sorts every key in all passes.  A onesweep
+// pass over a single tile is ~10 us of load, rank, publish and scatter, and
+// a small system's step is eight of them plus a launch gap each; here the
+// keys stay in registers and LDS between passes, a single tile's digit
+// offsets are a plain scan (no look-back), and only the last pass writes to
+// global memory (with the gather of pos_sorted).  Ranking is onesweep's:
+// per-wave ballot match, items in the same order, so the result is the same
+// stable sort.
+constexpr int kSmallThreads = 1024;
+template <typename K>
+struct SmallSort {
+  // 8 keys per thread for 64-bit keys (8192), 12 for 32-bit (12288; 16
+  // spilled registers at 1024 threads): LDS holds one copy of the keys and
+  // values plus the per-wave digit counters
+  static constexpr int kItems = sizeof(K) == 8 ? 8 : 12;
+  static constexpr int kMax = kSmallThreads * kItems;
+};
+
+template <typename K>
+__global__ __launch_bounds__(kSmallThreads) void small_sort_kernel(
+    const K* __restrict__ kin, K* __restrict__ kout, unsigned* __restrict__ vout, int n,
+    int passes, const float4* __restrict__ gsrc, float4* __restrict__ gdst) {
+  constexpr int ITEMS = SmallSort<K>::kItems;
+  constexpr int WAVES = kSmallThreads / 64;
+  __shared__ K s_key[SmallSort<K>::kMax];
+  __shared__ unsigned s_val[SmallSort<K>::kMax];
+  __shared__ unsigned s_cnt[WAVES][256];
+  __shared__ unsigned s_goff[256];
+  __shared__ unsigned s_scan[4];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int base = w * (64 * ITEMS);
+  K key[ITEMS];
+  unsigned val[ITEMS], rank[ITEMS];
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    const int idx = base + j * 64 + lane;
+    key[j] = idx < n ? kin[idx] : (K)~(K)0;  // padding sorts last and is never written
+    val[j] = (unsigned)idx;
+  }
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  for (int p = 0; p < passes; ++p) {
+    const int shift = p * 8;
+    for (int k = tid; k < WAVES * 256; k += kSmallThreads) (&s_cnt[0][0])[k] = 0u;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      const unsigned d = (unsigned)(key[j] >> shift) & 255u;
+      unsigned long long peers = ~0ull;
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const unsigned bit = (d >> b) & 1u;
+        const unsigned long long bal = __ballot(bit);
+        peers &= bit ? bal : ~bal;
+      }
+      const unsigned below = (unsigned)__popcll(peers & lt);
+      const unsigned old = s_cnt[w][d];
+      if (below == 0) s_cnt[w][d] = old + (unsigned)__popcll(peers);
+      rank[j] = old + below;
+    }
+    __syncthreads();
+    // digit `tid`: offsets of the waves within the digit, then the digits'
+    // exclusive scan (threads 0..255 are four waves)
+    if (tid < 256) {
+      unsigned total = 0;
+#pragma unroll
+      for (int k = 0; k < WAVES; ++k) {
+        const unsigned c = s_cnt[k][tid];
+        s_cnt[k][tid] = total;
+        total += c;
+      }
+      const unsigned inc = wave_incl_scan(total);
+      if (lane == 63) s_scan[w] = inc;
+      s_goff[tid] = inc - total;
+    }
+    __syncthreads();
+    if (tid < 256)
+      for (int k = 0; k < w; ++k) s_goff[tid] += s_scan[k];
+    __syncthreads();
+    const bool last = p == passes - 1;
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      const int idx = base + j * 64 + lane;
+      const unsigned d = (unsigned)(key[j] >> shift) & 255u;
+      const unsigned dest = s_goff[d] + s_cnt[w][d] + rank[j];
+      if (!last) {
+        s_key[dest] = key[j];
+        s_val[dest] = val[j];
+      } else if (val[j] < (unsigned)n && dest < (unsigned)n) {
+        // padding (original index >= n, key all ones) sorts after every real
+        // key in every pass, so the real keys fill [0, n)
+        kout[dest] = key[j];
+        vout[dest] = val[j];
+        if (gdst) gdst[dest] = gsrc[val[j]];
+      }
+    }
+    if (last) break;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      const int idx = base + j * 64 + lane;
+      key[j] = s_key[idx];
+      val[j] = s_val[idx];
+    }
+  }
+}
+
 int grid_for(int n, int per_thread, int cap) {
   long g = ((long)n + (long)kBlock * per_thread - 1) / ((long)kBlock * per_thread);
   if (g < 1) g = 1;
@@ -830,6 +936,20 @@ void launch_sort(hipStream_t s, int key_bits, void* keys_a, void* keys_b, unsign
                  int max_tiles, bool first_pass_identity, const float4* gather_src,
                  float4* gather_dst) {
   if (n <= 0) return;
+  // small inputs (identity values): one workgroup, all passes, result in the
+  // a buffers as after an even number of onesweep passes
+  if (first_pass_identity && key_bits == 30 && n <= SmallSort<uint32_t>::kMax) {
+    small_sort_kernel<uint32_t><<<1, kSmallThreads, 0, s>>>(
+        (const uint32_t*)keys_a, (uint32_t*)keys_a, vals_a, n, 4, gather_src, gather_dst);
+    NB_LAUNCH_CHECK();
+    return;
+  }
+  if (first_pass_identity && key_bits != 30 && n <= SmallSort<uint64_t>::kMax) {
+    small_sort_kernel<uint64_t><<<1, kSmallThreads, 0, s>>>(
+        (const uint64_t*)keys_a, (uint64_t*)keys_a, vals_a, n, 8, gather_src, gather_dst);
+    NB_LAUNCH_CHECK();
+    return;
+  }
   if (key_bits == 30)
     sort_typed<uint32_t>(s, (uint32_t*)keys_a, (uint32_t*)keys_b, vals_a, vals_b, n, ss, lookback,
                          max_tiles, first_pass_identity, gather_src, gather_dst, 4);
