@@ -71,13 +71,19 @@ __global__ __launch_bounds__(kBlock) void walk_thread_kernel(
   const int nm1 = n - 1;
   const float4 b = pos_sorted[t];
   float ax = 0.f, ay = 0.f, az = 0.f;
+  // The reference pushes both children and pops the right one first; here
+  // an opened node pushes only its left child and the walk continues straight
+  // into the right one, which is the same visit order without the stack
+  // round trip (the private stack lives in scratch memory).  An opened node
+  // then costs one dependent fetch, its 32-byte record as two 16-byte loads
+  // issued together, instead of stack pop, record, then children (1M
+  // particles: 8.45 -> 6.05 ms per force call).
   int stack[kStackDepth];
   int sp = 0, ni = 0, no = 0;
-  stack[sp++] = 0;
   const unsigned nodes_total = 2u * (unsigned)n - 1u;
   unsigned pops = 0;
-  while (sp > 0) {
-    const int id = stack[--sp];
+  int id = 0;
+  while (true) {
     if ((unsigned)id >= nodes_total || ++pops > nodes_total) {
       *error = 1;
       break;
@@ -90,23 +96,26 @@ __global__ __launch_bounds__(kBlock) void walk_thread_kernel(
       m = src.w;
     } else {
       const float4 c = nodes[id].com;
+      int4 meta = *reinterpret_cast<const int4*>(&nodes[id].left);  // left, right, size2, other
+      asm volatile("" : "+v"(meta.x), "+v"(meta.y), "+v"(meta.z), "+v"(meta.w));
       dx = c.x - b.x; dy = c.y - b.y; dz = c.z - b.z;
       d2 = dist2(dx, dy, dz);
       if (COUNT) ++no;
-      if (!(nodes[id].size2 < theta2 * d2)) {
-        if (sp + 2 > kStackDepth) {
+      if (!(__int_as_float(meta.z) < theta2 * d2)) {
+        if (sp + 1 > kStackDepth) {
           *overflow = 1;
           break;
         }
-        int2 ch = *reinterpret_cast<const int2*>(&nodes[id].left);
-        stack[sp++] = ch.x;
-        stack[sp++] = ch.y;
+        stack[sp++] = meta.x;
+        id = meta.y;
         continue;
       }
       m = c.w;
     }
     accumulate(dx, dy, dz, d2, m, eps2, ax, ay, az);
     if (COUNT) ++ni;
+    if (sp == 0) break;
+    id = stack[--sp];
   }
   if (COUNT) {
     counts[t] = ni;
