@@ -60,7 +60,25 @@ __device__ __forceinline__ void finish(int t, int n, float ax, float ay, float a
   }
 }
 
-template <bool COUNT>
+// Stack of the per-target walk: private (scratch memory) at large n, where
+// occupancy hides its latency, or one LDS column per thread ([depth][thread],
+// so lanes at the same depth use distinct banks) when the whole grid fits one
+// workgroup per CU and the walk is a single dependent chain per target.
+template <bool LDS>
+struct ThreadStack;
+template <>
+struct ThreadStack<false> {
+  int s[kStackDepth];
+  __device__ __forceinline__ int& operator[](int k) { return s[k]; }
+};
+template <>
+struct ThreadStack<true> {
+  int (*s)[kBlock];
+  __device__ __forceinline__ int& operator[](int k) { return s[k][threadIdx.x]; }
+};
+constexpr int kLdsStackMaxN = 256 * kBlock;  // one workgroup per CU of MI355X
+
+template <bool COUNT, bool LDS>
 __global__ __launch_bounds__(kBlock) void walk_thread_kernel(
     const float4* __restrict__ pos_sorted, const unsigned* __restrict__ perm, int n,
     const Node* __restrict__ nodes, float theta2, float eps2, float G, float4* __restrict__ acc,
@@ -78,7 +96,11 @@ __global__ __launch_bounds__(kBlock) void walk_thread_kernel(
   // then costs one dependent fetch, its 32-byte record as two 16-byte loads
   // issued together, instead of stack pop, record, then children (1M
   // particles: 8.45 -> 6.05 ms per force call).
-  int stack[kStackDepth];
+  ThreadStack<LDS> stack;
+  if constexpr (LDS) {
+    __shared__ int s_stack[kStackDepth][kBlock];
+    stack.s = s_stack;
+  }
   int sp = 0, ni = 0, no = 0;
   const unsigned nodes_total = 2u * (unsigned)n - 1u;
   unsigned pops = 0;
@@ -580,7 +602,10 @@ void walk_dispatch(hipStream_t s, int traverse, int width, const WalkArgs& a) {
   a.pos_sorted, a.perm, a.n, a.nodes, a.theta2, a.eps2, a.G, a.acc, a.vel, a.half_dt, a.counts, \
       a.opens, a.overflow, a.error
   if (traverse == 0) {
-    walk_thread_kernel<COUNT><<<g, kBlock, 0, s>>>(NB_WALK_ARGS);
+    if (a.n <= kLdsStackMaxN)
+      walk_thread_kernel<COUNT, true><<<g, kBlock, 0, s>>>(NB_WALK_ARGS);
+    else
+      walk_thread_kernel<COUNT, false><<<g, kBlock, 0, s>>>(NB_WALK_ARGS);
   } else if (width == 64 || width == 32 || width == 16) {
     const int groups = (a.n + width - 1) / width;
     const int gw = (groups + kBlock / 64 - 1) / (kBlock / 64);
