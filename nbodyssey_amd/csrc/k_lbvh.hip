@@ -756,8 +756,11 @@ constexpr int kSmallThreads = 1024;
 template <typename K>
 struct SmallSort {
   // 8 keys per thread for 64-bit keys (8192), 12 for 32-bit (12288; 16
-  // spilled registers at 1024 threads): LDS holds one copy of the keys and
-  // values plus the per-wave digit counters
+  // spilled registers at 1024 threads).  LDS holds one copy of the keys and
+  // values plus 16-bit per-wave digit counters.  Above ~8K keys one CU
+  // ranking every key is no faster than the onesweep passes (12000 63-bit
+  // keys: 96 us either way), and more keys per thread lengthen each wave's
+  // serial ranking (4096 keys: 42 us at 8 per thread, 52 us at 12).
   static constexpr int kItems = sizeof(K) == 8 ? 8 : 12;
   static constexpr int kMax = kSmallThreads * kItems;
 };
@@ -770,11 +773,16 @@ __global__ __launch_bounds__(kSmallThreads) void small_sort_kernel(
   constexpr int WAVES = kSmallThreads / 64;
   __shared__ K s_key[SmallSort<K>::kMax];
   __shared__ unsigned s_val[SmallSort<K>::kMax];
-  __shared__ unsigned s_cnt[WAVES][256];
+  __shared__ unsigned short s_cnt[WAVES][256];
   __shared__ unsigned s_goff[256];
   __shared__ unsigned s_scan[4];
+  static_assert(SmallSort<K>::kMax < 65536, "16-bit counters and offsets");
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int base = w * (64 * ITEMS);
+  // a wave whose slots are all padding ranks nothing: padding (all-ones
+  // keys) sorts after every real key in every pass, so it only fills the
+  // slots from n up and can be left out
+  const bool real_wave = base < n;
   K key[ITEMS];
   unsigned val[ITEMS], rank[ITEMS];
 #pragma unroll
@@ -786,10 +794,10 @@ __global__ __launch_bounds__(kSmallThreads) void small_sort_kernel(
   const unsigned long long lt = (1ull << lane) - 1ull;
   for (int p = 0; p < passes; ++p) {
     const int shift = p * 8;
-    for (int k = tid; k < WAVES * 256; k += kSmallThreads) (&s_cnt[0][0])[k] = 0u;
+    for (int k = tid; k < WAVES * 256; k += kSmallThreads) (&s_cnt[0][0])[k] = 0;
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < ITEMS; ++j) {
+    for (int j = 0; j < ITEMS && real_wave; ++j) {
       const unsigned d = (unsigned)(key[j] >> shift) & 255u;
       unsigned long long peers = ~0ull;
 #pragma unroll
@@ -800,7 +808,7 @@ __global__ __launch_bounds__(kSmallThreads) void small_sort_kernel(
       }
       const unsigned below = (unsigned)__popcll(peers & lt);
       const unsigned old = s_cnt[w][d];
-      if (below == 0) s_cnt[w][d] = old + (unsigned)__popcll(peers);
+      if (below == 0) s_cnt[w][d] = (unsigned short)(old + (unsigned)__popcll(peers));
       rank[j] = old + below;
     }
     __syncthreads();
@@ -811,7 +819,7 @@ __global__ __launch_bounds__(kSmallThreads) void small_sort_kernel(
 #pragma unroll
       for (int k = 0; k < WAVES; ++k) {
         const unsigned c = s_cnt[k][tid];
-        s_cnt[k][tid] = total;
+        s_cnt[k][tid] = (unsigned short)total;
         total += c;
       }
       const unsigned inc = wave_incl_scan(total);
@@ -824,8 +832,7 @@ __global__ __launch_bounds__(kSmallThreads) void small_sort_kernel(
     __syncthreads();
     const bool last = p == passes - 1;
 #pragma unroll
-    for (int j = 0; j < ITEMS; ++j) {
-      const int idx = base + j * 64 + lane;
+    for (int j = 0; j < ITEMS && real_wave; ++j) {
       const unsigned d = (unsigned)(key[j] >> shift) & 255u;
       const unsigned dest = s_goff[d] + s_cnt[w][d] + rank[j];
       if (!last) {
@@ -842,10 +849,12 @@ __global__ __launch_bounds__(kSmallThreads) void small_sort_kernel(
     if (last) break;
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < ITEMS; ++j) {
+    for (int j = 0; j < ITEMS && real_wave; ++j) {
       const int idx = base + j * 64 + lane;
-      key[j] = s_key[idx];
-      val[j] = s_val[idx];
+      // slots from n up were filled by padding, which skipped waves never
+      // wrote: restore it
+      key[j] = idx < n ? s_key[idx] : (K)~(K)0;
+      val[j] = idx < n ? s_val[idx] : (unsigned)idx;
     }
   }
 }
