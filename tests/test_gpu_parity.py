@@ -241,6 +241,43 @@ def test_reference_ic_energy_trace():
     assert abs(e[2] / -7.19439386e-01 - 1) < 2e-4
 
 
+@pytest.mark.parametrize("module,args,band", [
+    ("naive", ["--force", "naive"], 1.5),
+    ("bh_thread", ["--force", "bh", "--traverse", "thread"], 2.0),
+    ("bh_warp", ["--force", "bh", "--traverse", "warp"], 1.5),
+])
+def test_reference_energy_protocol(tmp_path, module, args, band):
+    """gpu_bench.sh:64-71 through galaxy_sim -- 12,000 generate_ic.py particles
+    (seed 0), dt 0.01, eps 0.05, 1,500 steps, a row every 100 -- against the
+    reference's committed logs (tests/golden/energy_gpu_*.csv): the same rows,
+    E0 to 1e-6, the post-step-0 energies to 1e-5, and a final relative energy
+    error within a factor `band` of the reference's (the Barnes-Hut drift comes
+    from the walk's force error, which the 32-lane walk shares)."""
+    import os
+    import subprocess
+    from nbodyssey_amd import GALAXY_SIM, formats
+    pos, vel = ic.generate_ic_disks(particles=12000, seed=0)
+    icp = tmp_path / "ic.bin"
+    formats.write_ic(icp, pos, vel)
+    elog = tmp_path / "energy.csv"
+    r = subprocess.run([GALAXY_SIM, "--ic", str(icp), "--steps", "1500", "--dump-every", "100",
+                        "--out", str(tmp_path / "frames"), "--energy-log", str(elog), *args],
+                       capture_output=True, text=True, timeout=120)
+    print(module, r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    ours = formats.read_energy_csv(elog)
+    ref = formats.read_energy_csv(os.path.join(os.path.dirname(__file__), "golden",
+                                               f"energy_gpu_{module}.csv"))
+    print(module, "final rel", ours["rel_error"][-1], "reference", ref["rel_error"][-1])
+    assert np.array_equal(ours["step"], ref["step"])
+    e0 = lambda log: log["energy"][0] / (1.0 - log["rel_error"][0])
+    assert abs(e0(ours) / e0(ref) - 1) < 1e-6
+    for col in ("kinetic", "potential"):
+        assert abs(ours[col][0] / ref[col][0] - 1) < 1e-5
+    ratio = ours["rel_error"][-1] / ref["rel_error"][-1]
+    assert 1 / band < ratio < band
+
+
 @pytest.mark.parametrize("walk", [("warp", 64), ("warp", 32), ("thread", 32)])
 def test_theta05_median_error_vs_fp64(walk):
     """theta = 0.5 median force error vs FP64 all-pairs (65,536 particles, 512

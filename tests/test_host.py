@@ -100,6 +100,56 @@ def test_energy_row_format():
     assert row == "100,1.00000,-1.17839655e-01,6.01599732e-01,-7.19439386e-01,-8.427e-07\n"
 
 
+# ---------------------------------------------- the reference's committed data
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+ENERGY_LOGS = ["energy_gpu_naive.csv", "energy_gpu_bh_thread.csv", "energy_gpu_bh_warp.csv"]
+
+
+def golden_e0(log):
+    """E0 from row 0 of an energy log: rel = (E - E0)/|E0| with E0 < 0."""
+    return log["energy"][0] / (1.0 - log["rel_error"][0])
+
+
+@pytest.mark.parametrize("name", ENERGY_LOGS)
+def test_reference_energy_logs_reformat(name):
+    """The reference's 12k logs (benchmarks/, gpu_bench.sh:64-71): rows after
+    steps 0, 100, ..., 1500 (main.cu:186-192), and our row writer reproduces
+    every row from its parsed values; rel_error agrees to the rounding of the
+    printed energies and rel_error's last printed digit."""
+    path = os.path.join(GOLDEN, name)
+    log = formats.read_energy_csv(path)
+    assert list(log["step"]) == list(range(0, 1501, 100))
+    e0 = golden_e0(log)
+    lines = open(path).read().splitlines()[1:]
+    for k, line in enumerate(lines):
+        mine = formats.energy_row(int(log["step"][k]), 0.01,
+                                  (log["energy"][k], log["kinetic"][k], log["potential"][k]),
+                                  (e0, 0.0, 0.0))
+        assert mine.rsplit(",", 1)[0] == line.rsplit(",", 1)[0]
+        # %.3e: one unit of the last digit, or the energies' own rounding
+        rel = log["rel_error"][k]
+        assert abs(float(mine.rsplit(",", 1)[1]) - rel) <= max(5e-8, 1.01e-3 * abs(rel))
+
+
+def test_reference_energy_logs_share_e0():
+    """The three logs start from one IC: their inverted E0 agree to 1e-8."""
+    e0 = [golden_e0(formats.read_energy_csv(os.path.join(GOLDEN, n))) for n in ENERGY_LOGS]
+    assert max(e0) - min(e0) < 1e-8 * abs(e0[0])
+
+
+def test_gpu_bench_covers_reference_sweep():
+    """python -m nbodyssey_amd.gpu_bench measures exactly the (n, method, theta)
+    points of the reference's gpu_results.csv, under the same header."""
+    from nbodyssey_amd import gpu_bench
+    lines = open(os.path.join(GOLDEN, "gpu_results.csv")).read().splitlines()
+    assert lines[0] == "n,method,theta,ms_per_call"
+    ref = {(int(n), m, float(t)) for n, m, t, _ in (l.split(",") for l in lines[1:])}
+    ours = {(n, m, 0.5) for n in gpu_bench.SIZES for m in ("naive", "bh-thread", "bh-warp")}
+    ours |= {(100000, m, t) for t in (0.3, 0.8) for m in ("bh-thread", "bh-warp")}
+    assert ours == ref
+
+
 # ------------------------------------------------------------- galaxy_sim CLI
 
 def run_cli(*args):

@@ -320,12 +320,18 @@ def test_energy_two_body_closed_form():
 
 
 def test_reference_trace_initial_energy():
-    """benchmarks/energy_gpu_naive.csv line 2: E = -1.17839549e-01 with
-    rel_error 5.740e-08 gives E0 = -0.11783955576; the restated generate_ic.py
-    IC (12,000 particles, seed 0) has the same energy to FP32 round-off."""
+    """benchmarks/energy_gpu_naive.csv (tests/golden) row 0: E = -1.17839549e-01
+    with rel_error 5.740e-08 gives E0 = -0.11783955576; the restated
+    generate_ic.py IC (12,000 particles, seed 0) has the same energy to FP32
+    round-off."""
+    from nbodyssey_amd import formats
+    import os
+    log = formats.read_energy_csv(os.path.join(os.path.dirname(__file__), "golden",
+                                               "energy_gpu_naive.csv"))
     pos, vel = ic.generate_ic_disks(particles=12000, seed=0)
     e = orc.energy_f64(pos, vel, 0.05)
-    e0_ref = -1.17839549e-01 / (1 - 5.740e-08)
+    e0_ref = log["energy"][0] / (1 - log["rel_error"][0])
+    assert abs(e0_ref / -0.11783955576 - 1) < 1e-9
     assert abs(e[0] / e0_ref - 1) < 1e-6
     # a wrong draw order (e.g. disks swapped) changes the energy by far more
     pos_b, vel_b = ic.generate_ic_disks(particles=12000, seed=1)
