@@ -90,7 +90,9 @@ int nb_set_params(nb_engine* h, const nb_params* p);
 int nb_get_params(const nb_engine* h, nb_params* p);
 int nb_count(const nb_engine* h);
 
-/* ParticleSystem::upload (src/simulation.cu:17-20).  vel4 may be NULL (zero). */
+/* ParticleSystem::upload (src/simulation.cu:17-20).  vel4 may be NULL (zero).
+ * Returns NB_ERR_INVALID, copying nothing, when a position/mass component or
+ * a velocity x/y/z is NaN or infinite (the reference copies such values). */
 int nb_upload(nb_engine* h, const float* pos4, const float* vel4);
 /* ParticleSystem::download_positions (src/simulation.cu:22-24); vel4 may be
  * NULL.  Waits for queued work. */

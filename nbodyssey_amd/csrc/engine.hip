@@ -829,6 +829,26 @@ void need(const void* ptr, const char* what) {
   if (!ptr) throw Error(kErrInvalid, std::string(what) + " is NULL");
 }
 
+// Rejects a host float4 array holding NaN or infinity in one of its first
+// `comps` components. Such a value would not fault a kernel (Morton keys are
+// clamped and every walk is bounded), but it reaches every force through the
+// tree's node masses and centres, so it is refused where the data enter.
+void need_finite(const float* v, size_t count, int comps, const char* what) {
+  const uint32_t kExp = 0x7f800000u;
+  uint32_t bad = 0;
+  for (size_t i = 0; i < count; ++i) {
+    uint32_t u[4];
+    std::memcpy(u, v + 4 * i, sizeof(u));
+    for (int c = 0; c < comps; ++c) bad |= (u[c] & kExp) == kExp;
+  }
+  if (!bad) return;
+  for (size_t i = 0; i < count; ++i)
+    for (int c = 0; c < comps; ++c)
+      if (!std::isfinite(v[4 * i + c]))
+        throw Error(kErrInvalid, std::string(what) + " holds a non-finite value at particle " +
+                                     std::to_string(i) + " component " + "xyzw"[c]);
+}
+
 // Host-buffer helpers for the stage-level entry points.
 template <typename T>
 void upload(DevBuf<T>& b, const T* h, size_t count) {
@@ -1033,6 +1053,8 @@ int nb_upload(nb_engine* h, const float* pos4, const float* vel4) {
     need(h, "engine");
     need(pos4, "pos4");
     nb::Engine& e = *h->e;
+    nb::need_finite(pos4, (size_t)e.n, 4, "pos4");
+    if (vel4) nb::need_finite(vel4, (size_t)e.n, 3, "vel4");
     size_t bytes = (size_t)e.n * sizeof(float4);
     NB_HIP(hipMemcpyAsync(e.pos, pos4, bytes, hipMemcpyHostToDevice, e.stream));
     if (vel4)

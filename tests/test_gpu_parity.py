@@ -620,3 +620,56 @@ def test_far_outlier(bits):
     print(f"outlier bits={bits}: distinct keys {len(np.unique(t['keys']))}, median err "
           f"{np.median(err):.2e}", flush=True)
     assert np.median(err) < 0.01 and np.all(np.isfinite(a))
+
+
+def test_upload_rejects_non_finite():
+    """nb_upload refuses a NaN or infinite position, mass or velocity (the
+    reference copies them, simulation.cu:17-20) and leaves the engine's state
+    as it was; velocity w is padding and is not checked."""
+    pos, vel = ic.two_disk_encounter(512, seed=3)
+    with engine.Simulation(pos, vel, force="bh", traverse="warp") as sim:
+        a0 = sim.compute_forces()
+        bad = pos.copy()
+        bad[5, 1] = np.nan
+        with pytest.raises(NbError, match="pos4 holds a non-finite value at particle 5 component y"):
+            sim.upload(bad, vel)
+        bad = pos.copy()
+        bad[511, 3] = -np.inf
+        with pytest.raises(NbError, match="particle 511 component w"):
+            sim.upload(bad, vel)
+        badv = vel.copy()
+        badv[0, 2] = np.inf
+        with pytest.raises(NbError, match="vel4 holds a non-finite value at particle 0 component z"):
+            sim.upload(pos, badv)
+        p, v = sim.download()
+        assert np.array_equal(p, pos) and np.array_equal(v[:, :3], vel[:, :3])
+        assert np.array_equal(sim.compute_forces(), a0)
+        padv = vel.copy()
+        padv[:, 3] = np.nan
+        sim.upload(pos, padv)
+        assert np.array_equal(sim.compute_forces(), a0)
+
+
+@pytest.mark.parametrize("traverse", ["thread", "warp"])
+def test_galaxy_sim_nan_particle_runs_to_end(tmp_path, traverse):
+    """The reference CLI copies an IC file as it is (main.cu:115-128), so one
+    NaN coordinate reaches every force through the tree.  The step still ends:
+    the NaN particle's Morton key clamps to cell 0, the tree stays valid, and
+    every node whose centre is NaN is opened, which adds at most the NaN
+    leaf's ancestors to each walk.  Energies come out NaN, as in the
+    reference."""
+    import subprocess
+    from nbodyssey_amd import GALAXY_SIM, formats
+    pos, vel = ic.generate_ic_disks(particles=3000, seed=5)
+    pos[17, 0] = np.nan
+    icp = tmp_path / "ic.bin"
+    formats.write_ic(icp, pos, vel)
+    elog = tmp_path / "energy.csv"
+    r = subprocess.run([GALAXY_SIM, "--ic", str(icp), "--steps", "4", "--dump-every", "2",
+                        "--out", str(tmp_path / "frames"), "--energy-log", str(elog), "--force",
+                        "bh", "--traverse", traverse], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    e = formats.read_energy_csv(elog)
+    assert list(e["step"]) == [0, 2, 4]
+    assert np.all(np.isnan(e["energy"]))
