@@ -17,6 +17,9 @@
 //                         all lanes (walk_group64 below).  Same visited /
 //                         opened / accepted sets, hence the same counts, as the
 //                         one-node-at-a-time warp vote.
+//   walk_split_kernel     group widths 32 and 16 in FP32: 64 targets per wave
+//                         in 2 or 4 subgroups that share the node batches but
+//                         keep their own votes and interaction lists.
 //   walk_items_kernel     the same walk over ensemble work items pulled from a
 //                         device queue (persistent).
 //   walk_potential_kernel the same walk accumulating the potential (tree energy).
@@ -566,6 +569,265 @@ __global__ __launch_bounds__(kBlock) void walk_wave_kernel(
                             s_stack[w], s_list[w], s_l16[w]);
 }
 
+// Group widths 32 and 16: S = 64 / width subgroups of consecutive targets
+// share one wave.  Each subgroup keeps the width-wide vote of
+// traverse_warp_kernel (a node opens for it iff one of its live lanes opens
+// the node) and its own interaction list, while the node batches are shared:
+// a stack entry carries the set of subgroups that still visit the node, so a
+// node is loaded and prefiltered once for all of them.  Every subgroup's
+// visited / opened / accepted sets, hence its counts, are those of its own
+// width-wide walk.  The S lists are evaluated together, each lane reading its
+// subgroup's list, so the list work follows the longest list instead of the
+// union of the lists that one 64-wide vote would evaluate.
+constexpr float kPadCoord = 1e18f;  // massless list padding, far from every target
+
+__device__ __forceinline__ float read_lane(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+template <bool COUNT, int S>
+__device__ __forceinline__ void walk_split64(
+    int t0, int count, const float4* __restrict__ pos_sorted, const unsigned* __restrict__ perm,
+    int n, const Node* __restrict__ nodes, float theta2, float eps2, float G,
+    float4* __restrict__ acc, float4* __restrict__ vel, float half_dt, int* __restrict__ counts,
+    int* __restrict__ opens, int* overflow, int* error, int* stk, unsigned char* smask,
+    float4* lists) {
+  static_assert(S == 2 || S == 4, "subgroups of 32 or 16 targets");
+  constexpr int W = 64 / S;
+  constexpr int R = kReps / S;  // representative targets per subgroup
+  constexpr unsigned long long kSubLanes = (1ull << W) - 1ull;
+  const int lane = threadIdx.x & 63;
+  const int sub = lane / W;
+  const bool live = lane < count;
+  const int t = t0 + (live ? lane : count - 1);
+  const int nm1 = n - 1;
+  const float4 b = pos_sorted[t];
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  const unsigned long long live_mask = __builtin_amdgcn_ballot_w64(live);
+
+  // subgroup boxes (xor offsets below W stay inside a subgroup; dead lanes
+  // repeat the last live target, which lies in their own subgroup whenever
+  // that subgroup is live), then every lane takes all S boxes and the
+  // representatives of every subgroup
+  float lx = b.x, ly = b.y, lz = b.z, hx = b.x, hy = b.y, hz = b.z;
+#pragma unroll
+  for (int o = W / 2; o > 0; o >>= 1) {
+    lx = fminf(lx, __shfl_xor(lx, o, 64));
+    ly = fminf(ly, __shfl_xor(ly, o, 64));
+    lz = fminf(lz, __shfl_xor(lz, o, 64));
+    hx = fmaxf(hx, __shfl_xor(hx, o, 64));
+    hy = fmaxf(hy, __shfl_xor(hy, o, 64));
+    hz = fmaxf(hz, __shfl_xor(hz, o, 64));
+  }
+  float blx[S], bly[S], blz[S], bhx[S], bhy[S], bhz[S];
+  float rx[S][R], ry[S][R], rz[S][R];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    blx[s] = read_lane(lx, s * W);
+    bly[s] = read_lane(ly, s * W);
+    blz[s] = read_lane(lz, s * W);
+    bhx[s] = read_lane(hx, s * W);
+    bhy[s] = read_lane(hy, s * W);
+    bhz[s] = read_lane(hz, s * W);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int src_lane = s * W + r * (W - 1) / (R - 1);
+      rx[s][r] = read_lane(b.x, src_lane);
+      ry[s][r] = read_lane(b.y, src_lane);
+      rz[s][r] = read_lane(b.z, src_lane);
+    }
+  }
+  const float acc_lo = theta2 * (1.f - kMacMargin);
+  const float acc_hi = theta2 * (1.f + kMacMargin);
+
+  float ax = 0.f, ay = 0.f, az = 0.f, cx = 0.f, cy = 0.f, cz = 0.f, pot = 0.f;
+  int sp = 1;
+  int nl[S];
+  long ni[S], no[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    nl[s] = 0;
+    ni[s] = no[s] = 0;
+  }
+  const unsigned nodes_total = 2u * (unsigned)n - 1u;
+  unsigned pops = 0;
+  if (lane == 0) {
+    stk[0] = 0;
+    smask[0] = (unsigned char)((1u << ((count + W - 1) / W)) - 1u);  // the live subgroups
+  }
+  while (sp > 0) {
+    // batches as in walk_group64; a node is pushed once, with the union of
+    // the subgroups that opened its parent, so it is popped at most once
+    int k = sp < 64 ? sp : 64;
+    if (sp > kWaveStack - kStackReserve) k = 1;
+    const bool valid = lane < k;
+    int id = 0;
+    unsigned m = 0;
+    if (valid) {
+      id = stk[sp - 1 - lane];
+      m = smask[sp - 1 - lane];
+    }
+    sp -= k;
+    pops += k;
+    if (__ballot(valid && (unsigned)id >= nodes_total) || pops > nodes_total) {
+      if (lane == 0) *error = 1;
+      break;
+    }
+    const bool is_leaf = valid && id >= nm1;
+    const bool is_node = valid && id < nm1;
+    float4 src = make_float4(0.f, 0.f, 0.f, 0.f);
+    float s2 = 0.f;
+    int left = 0, right = 0;
+    if (is_leaf) {
+      src = pos_sorted[id - nm1];
+    } else if (is_node) {
+      const Node nd = nodes[id];
+      src = nd.com;
+      s2 = nd.size2;
+      left = nd.left;
+      right = nd.right;
+    }
+    // per-subgroup prefilter, as in walk_group64 with the subgroup's box and
+    // representatives
+    unsigned open_bits = 0, amb_bits = 0;
+    if (is_node) {
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        const float ex = fmaxf(fmaxf(blx[s] - src.x, src.x - bhx[s]), 0.f);
+        const float ey = fmaxf(fmaxf(bly[s] - src.y, src.y - bhy[s]), 0.f);
+        const float ez = fmaxf(fmaxf(blz[s] - src.z, src.z - bhz[s]), 0.f);
+        const float dmin2 = ex * ex + ey * ey + ez * ez;
+        const float fx = fmaxf(fabsf(src.x - blx[s]), fabsf(src.x - bhx[s]));
+        const float fy = fmaxf(fabsf(src.y - bly[s]), fabsf(src.y - bhy[s]));
+        const float fz = fmaxf(fabsf(src.z - blz[s]), fabsf(src.z - bhz[s]));
+        const float dmax2 = fx * fx + fy * fy + fz * fz;
+        float rmin = dist2(src.x - rx[s][0], src.y - ry[s][0], src.z - rz[s][0]);
+#pragma unroll
+        for (int r = 1; r < R; ++r)
+          rmin = fminf(rmin, dist2(src.x - rx[s][r], src.y - ry[s][r], src.z - rz[s][r]));
+        const bool sure_accept = s2 < acc_lo * dmin2;
+        const bool sure_open = (s2 > acc_hi * dmax2) | !(s2 < theta2 * rmin);
+        open_bits |= (unsigned)sure_open << s;
+        amb_bits |= (unsigned)(!sure_accept & !sure_open) << s;
+      }
+      open_bits &= m;
+      amb_bits &= m;
+    }
+    // exact vote for the unsettled (node, subgroup) pairs: every lane tests
+    // each ambiguous node, and the ballot is split by subgroup
+    const unsigned long long amb = __builtin_amdgcn_ballot_w64(amb_bits != 0u);
+    if (amb) {
+      float4* vbuf = lists + 64;  // free upper half of list 0 (< 64 entries pending)
+      const int j = __popcll(amb & lt);
+      if (amb_bits) vbuf[j] = make_float4(src.x, src.y, src.z, s2);
+      const int na = __popcll(amb);
+      unsigned long long opened[S];
+#pragma unroll
+      for (int s = 0; s < S; ++s) opened[s] = 0;
+      for (int kk = 0; kk < na; kk += 2) {
+        const float4 v0 = vbuf[kk];
+        const float4 v1 = vbuf[kk + 1 < na ? kk + 1 : kk];
+        const bool op0 = !(v0.w < theta2 * dist2(v0.x - b.x, v0.y - b.y, v0.z - b.z));
+        const bool op1 = !(v1.w < theta2 * dist2(v1.x - b.x, v1.y - b.y, v1.z - b.z));
+        const unsigned long long b0 = __builtin_amdgcn_ballot_w64(op0) & live_mask;
+        const unsigned long long b1 = __builtin_amdgcn_ballot_w64(op1) & live_mask;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          if ((b0 >> (s * W)) & kSubLanes) opened[s] |= 1ull << kk;
+          if ((b1 >> (s * W)) & kSubLanes) opened[s] |= 2ull << kk;
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < S; ++s)
+        open_bits |= (((amb_bits >> s) & (unsigned)(opened[s] >> j)) & 1u) << s;
+    }
+    const unsigned acc_bits = is_leaf ? m : is_node ? (m & ~open_bits) : 0u;
+    if (COUNT) {
+#pragma unroll
+      for (int s = 0; s < S; ++s)
+        no[s] += __popcll(__builtin_amdgcn_ballot_w64(is_node && ((m >> s) & 1u)));
+    }
+    // push the children of nodes some subgroup opened, with those subgroups
+    const unsigned long long open_mask = __builtin_amdgcn_ballot_w64(open_bits != 0u);
+    const int nopen = __popcll(open_mask);
+    if (sp + 2 * nopen > kWaveStack) {
+      if (lane == 0) *overflow = 1;
+      break;
+    }
+    if (open_bits) {
+      const int slot = sp + 2 * __popcll(open_mask & lt);
+      stk[slot] = left;
+      stk[slot + 1] = right;
+      smask[slot] = (unsigned char)open_bits;
+      smask[slot + 1] = (unsigned char)open_bits;
+    }
+    sp += 2 * nopen;
+    // append each source to the list of every subgroup that accepts it
+    int most = 0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const unsigned long long am = __builtin_amdgcn_ballot_w64((acc_bits >> s) & 1u);
+      if ((am >> lane) & 1ull) lists[s * kWaveList + nl[s] + __popcll(am & lt)] = src;
+      nl[s] += __popcll(am);
+      if (COUNT) ni[s] += __popcll(am);
+      most = nl[s] > most ? nl[s] : most;
+    }
+    if (most >= 64) {
+      // shorter lists are padded with massless entries (an exact +0 per
+      // component), so all lanes evaluate 64 entries of their own list
+      const float4 pad = make_float4(kPadCoord, kPadCoord, kPadCoord, 0.f);
+#pragma unroll
+      for (int s = 0; s < S; ++s)
+        if (lane >= nl[s]) lists[s * kWaveList + lane] = pad;
+      flush_list<false, false>(lists + sub * kWaveList, 64, b.x, b.y, b.z, eps2, ax, ay, az, cx,
+                               cy, cz, pot);
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        nl[s] = nl[s] > 64 ? nl[s] - 64 : 0;
+        if (lane < nl[s]) lists[s * kWaveList + lane] = lists[s * kWaveList + 64 + lane];
+      }
+    }
+  }
+  int most = 0;
+#pragma unroll
+  for (int s = 0; s < S; ++s) most = nl[s] > most ? nl[s] : most;
+  if (most > 0) {
+    const float4 pad = make_float4(kPadCoord, kPadCoord, kPadCoord, 0.f);
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+      if (lane >= nl[s] && lane < most) lists[s * kWaveList + lane] = pad;
+    flush_list<false, false>(lists + sub * kWaveList, most, b.x, b.y, b.z, eps2, ax, ay, az, cx,
+                             cy, cz, pot);
+  }
+  if (COUNT) {
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+      if (lane == s * W && s * W < count) {
+        counts[t0 / W + s] = (int)ni[s];
+        opens[t0 / W + s] = (int)no[s];
+      }
+  }
+  if (live) finish(t, n, ax, ay, az, G, perm, acc, vel, half_dt);
+}
+
+template <bool COUNT, int S>
+__global__ __launch_bounds__(kBlock) void walk_split_kernel(
+    const float4* __restrict__ pos_sorted, const unsigned* __restrict__ perm, int n,
+    const Node* __restrict__ nodes, float theta2, float eps2, float G, float4* __restrict__ acc,
+    float4* __restrict__ vel, float half_dt, int* __restrict__ counts, int* __restrict__ opens,
+    int* overflow, int* error) {
+  constexpr int WAVES = kBlock / 64;
+  __shared__ int s_stack[WAVES][kWaveStack];
+  __shared__ unsigned char s_mask[WAVES][kWaveStack];
+  __shared__ float4 s_list[WAVES][S * kWaveList];
+  const int w = threadIdx.x >> 6;
+  const int gbase = xcd_block() * blockDim.x + (threadIdx.x & ~63);
+  if (gbase >= n) return;  // wave-uniform
+  walk_split64<COUNT, S>(gbase, min(64, n - gbase), pos_sorted, perm, n, nodes, theta2, eps2, G,
+                         acc, vel, half_dt, counts, opens, overflow, error, s_stack[w], s_mask[w],
+                         s_list[w]);
+}
+
 // Persistent ensemble walk: each wave pulls work items (system, first sorted
 // target, count <= 64) from a device queue until the queue is empty, walking
 // the system's own subtree (root = roots[system]) of the forest.  Every wave
@@ -613,6 +875,10 @@ void walk_dispatch(hipStream_t s, int traverse, int width, const WalkArgs& a) {
       walk_wave_kernel<COUNT, 2><<<gw, kBlock, 0, s>>>(NB_WALK_ARGS, width);
     else if (a.compensated && !COUNT)
       walk_wave_kernel<COUNT, 1><<<gw, kBlock, 0, s>>>(NB_WALK_ARGS, width);
+    else if (width == 32)
+      walk_split_kernel<COUNT, 2><<<g, kBlock, 0, s>>>(NB_WALK_ARGS);
+    else if (width == 16)
+      walk_split_kernel<COUNT, 4><<<g, kBlock, 0, s>>>(NB_WALK_ARGS);
     else
       walk_wave_kernel<COUNT, 0><<<gw, kBlock, 0, s>>>(NB_WALK_ARGS, width);
   } else {
