@@ -8,7 +8,9 @@
 // d2 = |com - target|^2 in the canonical FP32 order, strict, so theta = 0
 // never accepts.  Children are pushed left then right (right popped first).
 //
-//   walk_thread_kernel    one target per lane, private stack (traverse_kernel).
+//   walk_target_kernel    the per-target walk of traverse_kernel: every target
+//                         keeps its own decisions, and a wave walks its 64
+//                         targets together with a lane set per stack entry.
 //   walk_wave_kernel      the group walk of traverse_warp_kernel at group width
 //                         64, 32 or 16, one group per wave64: a node opens if
 //                         any live lane of the group opens it; node-parallel
@@ -61,92 +63,6 @@ __device__ __forceinline__ void finish(int t, int n, float ax, float ay, float a
     v.z = fmaf(half_dt, r.z, v.z);
     vel[o] = v;
   }
-}
-
-// Stack of the per-target walk: private (scratch memory) at large n, where
-// occupancy hides its latency, or one LDS column per thread ([depth][thread],
-// so lanes at the same depth use distinct banks) when the whole grid fits one
-// workgroup per CU and the walk is a single dependent chain per target.
-template <bool LDS>
-struct ThreadStack;
-template <>
-struct ThreadStack<false> {
-  int s[kStackDepth];
-  __device__ __forceinline__ int& operator[](int k) { return s[k]; }
-};
-template <>
-struct ThreadStack<true> {
-  int (*s)[kBlock];
-  __device__ __forceinline__ int& operator[](int k) { return s[k][threadIdx.x]; }
-};
-constexpr int kLdsStackMaxN = 256 * kBlock;  // one workgroup per CU of MI355X
-
-template <bool COUNT, bool LDS>
-__global__ __launch_bounds__(kBlock) void walk_thread_kernel(
-    const float4* __restrict__ pos_sorted, const unsigned* __restrict__ perm, int n,
-    const Node* __restrict__ nodes, float theta2, float eps2, float G, float4* __restrict__ acc,
-    float4* __restrict__ vel, float half_dt, int* __restrict__ counts, int* __restrict__ opens,
-    int* overflow, int* error) {
-  const int t = xcd_block() * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const int nm1 = n - 1;
-  const float4 b = pos_sorted[t];
-  float ax = 0.f, ay = 0.f, az = 0.f;
-  // The reference pushes both children and pops the right one first; here
-  // an opened node pushes only its left child and the walk continues straight
-  // into the right one, which is the same visit order without the stack
-  // round trip (the private stack lives in scratch memory).  An opened node
-  // then costs one dependent fetch, its 32-byte record as two 16-byte loads
-  // issued together, instead of stack pop, record, then children (1M
-  // particles: 8.45 -> 6.05 ms per force call).
-  ThreadStack<LDS> stack;
-  if constexpr (LDS) {
-    __shared__ int s_stack[kStackDepth][kBlock];
-    stack.s = s_stack;
-  }
-  int sp = 0, ni = 0, no = 0;
-  const unsigned nodes_total = 2u * (unsigned)n - 1u;
-  unsigned pops = 0;
-  int id = 0;
-  while (true) {
-    if ((unsigned)id >= nodes_total || ++pops > nodes_total) {
-      *error = 1;
-      break;
-    }
-    float dx, dy, dz, d2, m;
-    if (id >= nm1) {
-      float4 src = pos_sorted[id - nm1];
-      dx = src.x - b.x; dy = src.y - b.y; dz = src.z - b.z;
-      d2 = dist2(dx, dy, dz);
-      m = src.w;
-    } else {
-      const float4 c = nodes[id].com;
-      int4 meta = *reinterpret_cast<const int4*>(&nodes[id].left);  // left, right, size2, other
-      asm volatile("" : "+v"(meta.x), "+v"(meta.y), "+v"(meta.z), "+v"(meta.w));
-      dx = c.x - b.x; dy = c.y - b.y; dz = c.z - b.z;
-      d2 = dist2(dx, dy, dz);
-      if (COUNT) ++no;
-      if (!(__int_as_float(meta.z) < theta2 * d2)) {
-        if (sp + 1 > kStackDepth) {
-          *overflow = 1;
-          break;
-        }
-        stack[sp++] = meta.x;
-        id = meta.y;
-        continue;
-      }
-      m = c.w;
-    }
-    accumulate(dx, dy, dz, d2, m, eps2, ax, ay, az);
-    if (COUNT) ++ni;
-    if (sp == 0) break;
-    id = stack[--sp];
-  }
-  if (COUNT) {
-    counts[t] = ni;
-    opens[t] = no;
-  }
-  finish(t, n, ax, ay, az, G, perm, acc, vel, half_dt);
 }
 
 // Node-parallel wave64 group walk.  The visited / opened / accepted node sets
@@ -245,6 +161,220 @@ __device__ __forceinline__ void flush_half(const half2_t* l16, int npairs, half2
     fy = fmaf((float)uy.y, (float)w.y, fy);
     fz = fmaf((float)uz.y, (float)w.y, fz);
   }
+}
+
+// Per-target walk (traverse_kernel's semantics: every target its own
+// depth-first walk and its own open/accept decisions) run by one wave over 64
+// consecutive sorted targets.  A stack entry carries the set of lanes for
+// which all of the node's ancestors were opened, so the wave loads and tests
+// each node once for all of them, in batches of up to 64 nodes, instead of
+// every lane chasing its own chain of dependent node fetches.  Per node:
+//   * a leaf interacts with every lane of its set;
+//   * an internal node is settled for the whole set by the group box when
+//     size2 < theta2 * dmin2 (all accept) or size2 > theta2 * dmax2 (all
+//     open), each with the 1e-5 margin against FP32 rounding; otherwise every
+//     lane runs its own canonical test and the ballot is the set that opens;
+//   * the lanes that open it push its two children with that set.
+// Children are pushed in pairs that share one set (the root shares its pair
+// with a null entry), so a set is stored per pair.  Accepted nodes and leaves
+// go to an interaction list with their sets; each lane adds only the entries
+// of its own set (the others contribute an exact zero), so every target sums
+// exactly its own interaction list, in batch order.
+__device__ __forceinline__ void flush_masked(const float4* list, const unsigned* mw, int cnt,
+                                             int sh, float bx, float by, float bz, float eps2,
+                                             float& ax, float& ay, float& az, int& taken) {
+#pragma unroll 8
+  for (int k = 0; k < cnt; ++k) {
+    const float4 s = list[k];
+    const int keep = (int)(mw[k] << sh) >> 31;  // all ones iff this lane is in the set
+    const float dx = s.x - bx, dy = s.y - by, dz = s.z - bz;
+    const float inv = rsq(fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2))));
+    const float f = __int_as_float(__float_as_int(s.w * inv * inv * inv) & keep);
+    ax = fmaf(dx, f, ax);
+    ay = fmaf(dy, f, ay);
+    az = fmaf(dz, f, az);
+    taken -= keep;
+  }
+}
+
+template <bool COUNT>
+__device__ __forceinline__ void walk_target64(
+    int t0, int count, const float4* __restrict__ pos_sorted, const unsigned* __restrict__ perm,
+    int n, const Node* __restrict__ nodes, float theta2, float eps2, float G,
+    float4* __restrict__ acc, float4* __restrict__ vel, float half_dt, int* __restrict__ counts,
+    int* __restrict__ opens, int* overflow, int* error, int* stk, unsigned long long* smask,
+    float4* list, unsigned* mlo, unsigned* mhi) {
+  const int lane = threadIdx.x & 63;
+  const bool live = lane < count;
+  const int t = t0 + (live ? lane : count - 1);
+  const int nm1 = n - 1;
+  const float4 b = pos_sorted[t];
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  const unsigned long long live_mask = __builtin_amdgcn_ballot_w64(live);
+  const float blx = wave_min(b.x), bly = wave_min(b.y), blz = wave_min(b.z);
+  const float bhx = wave_max(b.x), bhy = wave_max(b.y), bhz = wave_max(b.z);
+  const float acc_lo = theta2 * (1.f - kMacMargin);
+  const float acc_hi = theta2 * (1.f + kMacMargin);
+  // this lane's word of a 64-bit set, shifted so that its bit is the sign bit
+  const unsigned* mw = lane < 32 ? mlo : mhi;
+  const int sh = 31 - (lane & 31);
+
+  float ax = 0.f, ay = 0.f, az = 0.f;
+  int taken = 0, tested = 0;
+  int sp = 2, nlist = 0;
+  const unsigned nodes_total = 2u * (unsigned)n - 1u;
+  unsigned pops = 0;
+  if (lane == 0) {
+    stk[0] = 0;   // root
+    stk[1] = -1;  // null partner
+    smask[0] = live_mask;
+  }
+  while (sp > 0) {
+    // sp stays even: pairs are popped whole (near the top of the stack one
+    // pair at a time, two entries per tree level, so it cannot overflow)
+    int k = sp < 64 ? sp : 64;
+    if (sp > kWaveStack - kStackReserve) k = 2;
+    const bool in_batch = lane < k;
+    int id = -1;
+    unsigned long long m = 0ull;
+    if (in_batch) {
+      const int slot = sp - 1 - lane;
+      id = stk[slot];
+      m = smask[slot >> 1];
+    }
+    sp -= k;
+    pops += k;  // every node is pushed once, by its parent, with one set
+    const bool valid = in_batch && id >= 0;
+    if (__builtin_amdgcn_ballot_w64(valid && (unsigned)id >= nodes_total) ||
+        pops > nodes_total + 1u) {
+      if (lane == 0) *error = 1;
+      break;
+    }
+    const bool is_leaf = valid && id >= nm1;
+    const bool is_node = valid && id < nm1;
+    float4 src = make_float4(0.f, 0.f, 0.f, 0.f);
+    float s2 = 0.f;
+    int left = 0, right = 0;
+    if (is_leaf) {
+      src = pos_sorted[id - nm1];
+    } else if (is_node) {
+      const Node nd = nodes[id];
+      src = nd.com;
+      s2 = nd.size2;
+      left = nd.left;
+      right = nd.right;
+    }
+    bool all_open = false, amb = false;
+    if (is_node) {
+      const float ex = fmaxf(fmaxf(blx - src.x, src.x - bhx), 0.f);
+      const float ey = fmaxf(fmaxf(bly - src.y, src.y - bhy), 0.f);
+      const float ez = fmaxf(fmaxf(blz - src.z, src.z - bhz), 0.f);
+      const float dmin2 = ex * ex + ey * ey + ez * ez;
+      const float fx = fmaxf(fabsf(src.x - blx), fabsf(src.x - bhx));
+      const float fy = fmaxf(fabsf(src.y - bly), fabsf(src.y - bhy));
+      const float fz = fmaxf(fabsf(src.z - blz), fabsf(src.z - bhz));
+      const float dmax2 = fx * fx + fy * fy + fz * fz;
+      const bool all_acc = s2 < acc_lo * dmin2;
+      all_open = s2 > acc_hi * dmax2;
+      amb = !(all_acc || all_open);
+    }
+    unsigned long long om = all_open ? m : 0ull;  // lanes of the set that open the node
+    const unsigned long long ambm = __builtin_amdgcn_ballot_w64(amb);
+    if (ambm) {
+      // per-lane canonical tests of the unsettled nodes, staged in the free
+      // upper halves of the list (< 64 entries pending), two per iteration;
+      // the ballots go back through the free upper halves of the set words
+      float4* vbuf = list + 64;
+      const int j = __popcll(ambm & lt);
+      if (amb) vbuf[j] = make_float4(src.x, src.y, src.z, s2);
+      const int na = __popcll(ambm);
+      for (int kk = 0; kk < na; kk += 2) {
+        const float4 v0 = vbuf[kk];
+        const float4 v1 = vbuf[kk + 1 < na ? kk + 1 : kk];
+        const bool op0 = !(v0.w < theta2 * dist2(v0.x - b.x, v0.y - b.y, v0.z - b.z));
+        const bool op1 = !(v1.w < theta2 * dist2(v1.x - b.x, v1.y - b.y, v1.z - b.z));
+        const unsigned long long b0 = __builtin_amdgcn_ballot_w64(op0);
+        const unsigned long long b1 = __builtin_amdgcn_ballot_w64(op1);
+        if (lane == 0) {
+          mlo[64 + kk] = (unsigned)b0;
+          mhi[64 + kk] = (unsigned)(b0 >> 32);
+          mlo[65 + kk] = (unsigned)b1;  // kk + 1 < 64: na <= 64 and kk is even
+          mhi[65 + kk] = (unsigned)(b1 >> 32);
+        }
+      }
+      if (amb) om = (((unsigned long long)mhi[64 + j] << 32) | mlo[64 + j]) & m;
+    }
+    const unsigned long long am = is_leaf ? m : is_node ? (m & ~om) : 0ull;
+    if (COUNT) {
+      // opening tests: internal nodes of the batch whose set holds this lane
+      const unsigned long long nm = __builtin_amdgcn_ballot_w64(is_node);
+      if (is_node) {
+        const int r = __popcll(nm & lt);
+        mlo[64 + r] = (unsigned)m;
+        mhi[64 + r] = (unsigned)(m >> 32);
+      }
+      const int nn = __popcll(nm);
+      for (int r = 0; r < nn; ++r) tested += (int)((mw[64 + r] << sh) >> 31);
+    }
+    // push the children of nodes some lane opened, as a pair with that set
+    const unsigned long long pm = __builtin_amdgcn_ballot_w64(om != 0ull);
+    const int npush = __popcll(pm);
+    if (sp + 2 * npush > kWaveStack) {
+      if (lane == 0) *overflow = 1;
+      break;
+    }
+    if (om) {
+      const int slot = sp + 2 * __popcll(pm & lt);
+      stk[slot] = left;
+      stk[slot + 1] = right;
+      smask[slot >> 1] = om;
+    }
+    sp += 2 * npush;
+    // append accepted nodes and leaves with their sets (lane order)
+    const unsigned long long apm = __builtin_amdgcn_ballot_w64(am != 0ull);
+    if (am) {
+      const int e = nlist + __popcll(apm & lt);
+      list[e] = src;
+      mlo[e] = (unsigned)am;
+      mhi[e] = (unsigned)(am >> 32);
+    }
+    nlist += __popcll(apm);
+    if (nlist >= 64) {
+      flush_masked(list, mw, 64, sh, b.x, b.y, b.z, eps2, ax, ay, az, taken);
+      nlist -= 64;
+      if (lane < nlist) {
+        list[lane] = list[64 + lane];
+        mlo[lane] = mlo[64 + lane];
+        mhi[lane] = mhi[64 + lane];
+      }
+    }
+  }
+  if (nlist > 0) flush_masked(list, mw, nlist, sh, b.x, b.y, b.z, eps2, ax, ay, az, taken);
+  if (COUNT && live) {
+    counts[t] = taken;
+    opens[t] = tested;
+  }
+  if (live) finish(t, n, ax, ay, az, G, perm, acc, vel, half_dt);
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(kBlock) void walk_target_kernel(
+    const float4* __restrict__ pos_sorted, const unsigned* __restrict__ perm, int n,
+    const Node* __restrict__ nodes, float theta2, float eps2, float G, float4* __restrict__ acc,
+    float4* __restrict__ vel, float half_dt, int* __restrict__ counts, int* __restrict__ opens,
+    int* overflow, int* error) {
+  constexpr int WAVES = kBlock / 64;
+  __shared__ int s_stack[WAVES][kWaveStack];
+  __shared__ unsigned long long s_smask[WAVES][kWaveStack / 2];
+  __shared__ float4 s_list[WAVES][kWaveList];
+  __shared__ unsigned s_mlo[WAVES][kWaveList];
+  __shared__ unsigned s_mhi[WAVES][kWaveList];
+  const int w = threadIdx.x >> 6;
+  const int gbase = xcd_block() * blockDim.x + (threadIdx.x & ~63);
+  if (gbase >= n) return;  // wave-uniform
+  walk_target64<COUNT>(gbase, min(64, n - gbase), pos_sorted, perm, n, nodes, theta2, eps2, G,
+                       acc, vel, half_dt, counts, opens, overflow, error, s_stack[w], s_smask[w],
+                       s_list[w], s_mlo[w], s_mhi[w]);
 }
 
 // One group of `count` (<= 64) consecutive sorted targets starting at t0,
@@ -580,6 +710,11 @@ __global__ __launch_bounds__(kBlock) void walk_wave_kernel(
 // subgroup's list, so the list work follows the longest list instead of the
 // union of the lists that one 64-wide vote would evaluate.
 constexpr float kPadCoord = 1e18f;  // massless list padding, far from every target
+// The split walk runs half (width 32) or a quarter (width 16) as many waves as
+// one group per wave, which loses while the grid does not fill the GPU: at
+// width 32 one group per wave is faster up to 400K targets and slower from
+// 600K on (1M: 1.29 -> 1.20 ms per step).
+constexpr int kSplitMinN = 1 << 19;
 
 __device__ __forceinline__ float read_lane(float v, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
@@ -864,10 +999,7 @@ void walk_dispatch(hipStream_t s, int traverse, int width, const WalkArgs& a) {
   a.pos_sorted, a.perm, a.n, a.nodes, a.theta2, a.eps2, a.G, a.acc, a.vel, a.half_dt, a.counts, \
       a.opens, a.overflow, a.error
   if (traverse == 0) {
-    if (a.n <= kLdsStackMaxN)
-      walk_thread_kernel<COUNT, true><<<g, kBlock, 0, s>>>(NB_WALK_ARGS);
-    else
-      walk_thread_kernel<COUNT, false><<<g, kBlock, 0, s>>>(NB_WALK_ARGS);
+    walk_target_kernel<COUNT><<<g, kBlock, 0, s>>>(NB_WALK_ARGS);
   } else if (width == 64 || width == 32 || width == 16) {
     const int groups = (a.n + width - 1) / width;
     const int gw = (groups + kBlock / 64 - 1) / (kBlock / 64);
@@ -875,9 +1007,9 @@ void walk_dispatch(hipStream_t s, int traverse, int width, const WalkArgs& a) {
       walk_wave_kernel<COUNT, 2><<<gw, kBlock, 0, s>>>(NB_WALK_ARGS, width);
     else if (a.compensated && !COUNT)
       walk_wave_kernel<COUNT, 1><<<gw, kBlock, 0, s>>>(NB_WALK_ARGS, width);
-    else if (width == 32)
+    else if (width == 32 && a.n >= kSplitMinN)
       walk_split_kernel<COUNT, 2><<<g, kBlock, 0, s>>>(NB_WALK_ARGS);
-    else if (width == 16)
+    else if (width == 16 && a.n >= kSplitMinN)
       walk_split_kernel<COUNT, 4><<<g, kBlock, 0, s>>>(NB_WALK_ARGS);
     else
       walk_wave_kernel<COUNT, 0><<<gw, kBlock, 0, s>>>(NB_WALK_ARGS, width);
