@@ -291,6 +291,11 @@ int chunk_for(int n, int split) {
 
 }  // namespace
 
+void warm_allpairs() {
+  hipFuncAttributes a;
+  NB_HIP(hipFuncGetAttributes(&a, reinterpret_cast<const void*>(energy_kernel)));
+}
+
 int allpairs_split(int n) { return split_for((n + kSpan - 1) / kSpan, n); }
 
 void launch_allpairs(hipStream_t s, const float4* pos, int n, float eps2, float G, float4* acc,

@@ -296,6 +296,11 @@ __global__ void ens_box_init_kernel(unsigned* box, int nsys) {
 
 }  // namespace
 
+void warm_ensemble() {
+  hipFuncAttributes a;
+  NB_HIP(hipFuncGetAttributes(&a, reinterpret_cast<const void*>(ens_box_init_kernel)));
+}
+
 int ens_passes() { return kEnsPasses; }
 int ens_max_systems() { return kEnsMaxSys; }
 

@@ -115,6 +115,11 @@ __global__ void fill_u32_kernel(unsigned* p, unsigned v, size_t count) {
 
 }  // namespace
 
+void warm_integrate() {
+  hipFuncAttributes a;
+  NB_HIP(hipFuncGetAttributes(&a, reinterpret_cast<const void*>(kick_drift_bbox_kernel)));
+}
+
 int bbox_parts(int n) {
   int g = (n + kBlock - 1) / kBlock;
   return g < kBboxBlocks ? (g > 0 ? g : 1) : kBboxBlocks;

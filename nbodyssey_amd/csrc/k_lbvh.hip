@@ -905,6 +905,11 @@ void sort_typed(hipStream_t s, K* ka, K* kb, unsigned* va, unsigned* vb, int n, 
 
 }  // namespace
 
+void warm_lbvh() {
+  hipFuncAttributes a;
+  NB_HIP(hipFuncGetAttributes(&a, reinterpret_cast<const void*>(com_local_kernel)));
+}
+
 int sort_passes(int key_bits) { return key_bits == 30 ? 4 : 8; }
 
 void launch_morton_hist(hipStream_t s, const float4* pos, int n, const float4* partial,

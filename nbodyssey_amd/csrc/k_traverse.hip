@@ -1022,6 +1022,11 @@ void walk_dispatch(hipStream_t s, int traverse, int width, const WalkArgs& a) {
 
 }  // namespace
 
+void warm_traverse() {
+  hipFuncAttributes a;
+  NB_HIP(hipFuncGetAttributes(&a, reinterpret_cast<const void*>(walk_wave_kernel<false, 0>)));
+}
+
 // Resident workgroups of the persistent walk: CUs x occupancy, queried once.
 static int walk_items_grid() {
   static int grid = 0;

@@ -147,6 +147,15 @@ void launch_com(hipStream_t s, const float4* pos_sorted, int n, Node* nodes, con
                 unsigned* visit, float4* box_lo, float4* box_hi, int* scratch, int* error);
 
 // k_traverse.hip
+// Each kernel translation unit's code object is loaded by the runtime at its
+// first use; these resolve one kernel of each so that the loading happens when
+// an engine is created rather than inside the first timed force call.
+void warm_integrate();
+void warm_allpairs();
+void warm_lbvh();
+void warm_traverse();
+void warm_ensemble();
+
 struct WalkArgs {
   const float4* pos_sorted;
   const unsigned* perm;
