@@ -447,6 +447,13 @@ def bh_main(args, pos, vel, eps, dt, what, metric, data, energy_theta=None, cpu=
     cpu_line = cpu_baseline(n, args.cpu_sample, eps, dt) if cpu else None
 
     names = ["bbox", "morton", "sort", "build", "com", "traverse"]
+    build = build_roofline(phase, calls, n, args.key_bits)
+    # whole-step roofline: each phase at the larger of its HBM time (algorithmic
+    # bytes at 8 TB/s) and its FP32 time (algorithmic flops at 157.3 TFLOP/s);
+    # the walk streams pos_sorted, perm and acc (36 B per particle)
+    t_roof_ms = 1e3 * (sum(v["bytes_per_particle"] for v in build.values()) * n
+                       / (HBM_PEAK_GBS * 1e9)
+                       + max(36.0 * n / (HBM_PEAK_GBS * 1e9), flops / (FP32_PEAK_TFLOPS * 1e12)))
     out = {
         "metric": metric,
         "value": value,
@@ -466,7 +473,11 @@ def bh_main(args, pos, vel, eps, dt, what, metric, data, energy_theta=None, cpu=
                    "n": n, "theta": args.theta, "key_bits": args.key_bits,
                    "traverse": args.traverse, "group_width": width},
         "phase_ms": {k: phase[i] / max(calls, 1) for i, k in enumerate(names)},
-        "build_roofline_hbm": build_roofline(phase, calls, n, args.key_bits),
+        "build_roofline_hbm": build,
+        "step_roofline": {"t_roof_ms": t_roof_ms, "t_meas_ms": ms_per_step,
+                          "frac": t_roof_ms / ms_per_step,
+                          "convention": "sum over phases of max(algorithmic bytes / 8 TB/s, "
+                                        "per-target-walk flops / 157.3 TFLOP/s)"},
         "roofline": {"bound": "valu", "kernel": "walk",
                      "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": profiled_walk_traffic(args),
