@@ -85,6 +85,12 @@ __device__ __forceinline__ void finish(int t, int n, float ax, float ay, float a
 // blocks of 64 (LDS broadcast reads, pure FP32 VALU), optionally Kahan-adding
 // each block's partial sums (compensated mode).
 constexpr int kWaveStack = 1024;   // LDS stack entries per wave
+// Per-target walk: 768 stack entries (3 KB, plus 3 KB of lane sets) let four
+// workgroups share a CU instead of three.  Its stack peaks near 260 entries at
+// theta 0.5 and 500 at theta 0.3 on 1M particles (1M step: 1.90 -> 1.74 ms at
+// theta 0.5, 3.94 -> 3.90 ms at 0.3); 640 entries push theta 0.3 into the
+// pair-at-a-time mode near the top (6.0 ms).
+constexpr int kTargetStack = 768;
 constexpr int kStackReserve = 64 + 2 * kStackDepth;
 constexpr int kWaveList = 128;     // LDS interaction-list entries per wave
 constexpr float kMacMargin = 1e-5f;
@@ -233,7 +239,7 @@ __device__ __forceinline__ void walk_target64(
     // sp stays even: pairs are popped whole (near the top of the stack one
     // pair at a time, two entries per tree level, so it cannot overflow)
     int k = sp < 64 ? sp : 64;
-    if (sp > kWaveStack - kStackReserve) k = 2;
+    if (sp > kTargetStack - kStackReserve) k = 2;
     const bool in_batch = lane < k;
     int id = -1;
     unsigned long long m = 0ull;
@@ -319,7 +325,7 @@ __device__ __forceinline__ void walk_target64(
     // push the children of nodes some lane opened, as a pair with that set
     const unsigned long long pm = __builtin_amdgcn_ballot_w64(om != 0ull);
     const int npush = __popcll(pm);
-    if (sp + 2 * npush > kWaveStack) {
+    if (sp + 2 * npush > kTargetStack) {
       if (lane == 0) *overflow = 1;
       break;
     }
@@ -364,8 +370,8 @@ __global__ __launch_bounds__(kBlock) void walk_target_kernel(
     float4* __restrict__ vel, float half_dt, int* __restrict__ counts, int* __restrict__ opens,
     int* overflow, int* error) {
   constexpr int WAVES = kBlock / 64;
-  __shared__ int s_stack[WAVES][kWaveStack];
-  __shared__ unsigned long long s_smask[WAVES][kWaveStack / 2];
+  __shared__ int s_stack[WAVES][kTargetStack];
+  __shared__ unsigned long long s_smask[WAVES][kTargetStack / 2];
   __shared__ float4 s_list[WAVES][kWaveList];
   __shared__ unsigned s_mlo[WAVES][kWaveList];
   __shared__ unsigned s_mhi[WAVES][kWaveList];
