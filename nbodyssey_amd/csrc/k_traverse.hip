@@ -19,10 +19,11 @@
 //                         all lanes (walk_group64 below).  Same visited /
 //                         opened / accepted sets, hence the same counts, as the
 //                         one-node-at-a-time warp vote.
-//   walk_split_kernel     group widths 32 and 16 in FP32: 64 targets per wave
-//                         in 2 or 4 subgroups that share the node batches but
-//                         keep their own votes and interaction lists.
-//   walk_items_kernel     the same walk over ensemble work items pulled from a
+//   walk_split_kernel     group widths 32 and 16 in FP32 from 2^19 targets on:
+//                         64 targets per wave in 2 or 4 subgroups that share
+//                         the node batches but keep their own votes and
+//                         interaction lists.
+//   walk_items_kernel     the wave64 group walk over ensemble work items pulled from a
 //                         device queue (persistent).
 //   walk_potential_kernel the same walk accumulating the potential (tree energy).
 // COUNT variants also record interactions and opening tests (per target for
