@@ -259,18 +259,16 @@ __device__ __forceinline__ void walk_target64(
     }
     const bool is_leaf = valid && id >= nm1;
     const bool is_node = valid && id < nm1;
+    // (x, y, z, m) of the leaf or node and the node's (left, right, size2):
+    // two loads issued together (with the leaf and node loads in separate
+    // branches the leaf load waited for the node load's registers)
+    const float4* srcp = is_leaf ? pos_sorted + (id - nm1) : &nodes[is_node ? id : 0].com;
     float4 src = make_float4(0.f, 0.f, 0.f, 0.f);
-    float s2 = 0.f;
-    int left = 0, right = 0;
-    if (is_leaf) {
-      src = pos_sorted[id - nm1];
-    } else if (is_node) {
-      const Node nd = nodes[id];
-      src = nd.com;
-      s2 = nd.size2;
-      left = nd.left;
-      right = nd.right;
-    }
+    if (valid) src = *srcp;
+    int4 meta = make_int4(0, 0, 0, 0);
+    if (is_node) meta = *reinterpret_cast<const int4*>(&nodes[id].left);
+    const float s2 = __int_as_float(meta.z);
+    const int left = meta.x, right = meta.y;
     bool all_open = false, amb = false;
     if (is_node) {
       const float ex = fmaxf(fmaxf(blx - src.x, src.x - bhx), 0.f);
@@ -467,18 +465,16 @@ __device__ __forceinline__ void walk_group64(
     }
     const bool is_leaf = valid && id >= nm1;
     const bool is_node = valid && id < nm1;
+    // (x, y, z, m) of the leaf or node and the node's (left, right, size2):
+    // two loads issued together (with the leaf and node loads in separate
+    // branches the leaf load waited for the node load's registers)
+    const float4* srcp = is_leaf ? pos_sorted + (id - nm1) : &nodes[is_node ? id : 0].com;
     float4 src = make_float4(0.f, 0.f, 0.f, 0.f);
-    float s2 = 0.f;
-    int left = 0, right = 0;
-    if (is_leaf) {
-      src = pos_sorted[id - nm1];
-    } else if (is_node) {
-      const Node nd = nodes[id];
-      src = nd.com;
-      s2 = nd.size2;
-      left = nd.left;
-      right = nd.right;
-    }
+    if (valid) src = *srcp;
+    int4 meta = make_int4(0, 0, 0, 0);
+    if (is_node) meta = *reinterpret_cast<const int4*>(&nodes[id].left);
+    const float s2 = __int_as_float(meta.z);
+    const int left = meta.x, right = meta.y;
     bool sure_open = false, ambiguous = false, safe16 = false;
     if (is_node) {
       const float ex = fmaxf(fmaxf(blx - src.x, src.x - bhx), 0.f);
@@ -817,18 +813,16 @@ __device__ __forceinline__ void walk_split64(
     }
     const bool is_leaf = valid && id >= nm1;
     const bool is_node = valid && id < nm1;
+    // (x, y, z, m) of the leaf or node and the node's (left, right, size2):
+    // two loads issued together (with the leaf and node loads in separate
+    // branches the leaf load waited for the node load's registers)
+    const float4* srcp = is_leaf ? pos_sorted + (id - nm1) : &nodes[is_node ? id : 0].com;
     float4 src = make_float4(0.f, 0.f, 0.f, 0.f);
-    float s2 = 0.f;
-    int left = 0, right = 0;
-    if (is_leaf) {
-      src = pos_sorted[id - nm1];
-    } else if (is_node) {
-      const Node nd = nodes[id];
-      src = nd.com;
-      s2 = nd.size2;
-      left = nd.left;
-      right = nd.right;
-    }
+    if (valid) src = *srcp;
+    int4 meta = make_int4(0, 0, 0, 0);
+    if (is_node) meta = *reinterpret_cast<const int4*>(&nodes[id].left);
+    const float s2 = __int_as_float(meta.z);
+    const int left = meta.x, right = meta.y;
     // per-subgroup prefilter, as in walk_group64 with the subgroup's box and
     // representatives
     unsigned open_bits = 0, amb_bits = 0;
