@@ -76,9 +76,12 @@ __device__ __forceinline__ void finish(int t, int n, float ax, float ay, float a
 // loads node k (independent vector loads), and the open/accept vote of each
 // node over the 64 targets is settled by
 //   * a prefilter on the group's bounding box: a node is accepted by every
-//     lane if size2 < theta2 * dmin2 (dmin = COM-to-box distance), opened by
-//     every lane if size2 >= theta2 * dmax2 (farthest box corner), each with
-//     a 1e-5 relative margin so FP32 rounding cannot flip a lane's test;
+//     lane if size2 < theta2 * dmin2 (dmin = COM-to-box distance, with a
+//     1e-5 relative margin so FP32 rounding cannot flip a lane's test);
+//   * representative targets (below): one whose own canonical test opens the
+//     node opens it for the group.  A farthest-corner test ("every lane
+//     opens") adds nothing here: every representative lies in the box, so
+//     whenever it holds a representative opens the node too;
 //   * otherwise the exact per-lane test in the canonical order, node by node
 //     with the node broadcast from LDS, reduced by ballot.
 // Opened nodes push their two children; accepted nodes and leaves append
@@ -417,7 +420,6 @@ __device__ __forceinline__ void walk_group64(
   float blx = wave_min(b.x), bly = wave_min(b.y), blz = wave_min(b.z);
   float bhx = wave_max(b.x), bhy = wave_max(b.y), bhz = wave_max(b.z);
   const float acc_lo = theta2 * (1.f - kMacMargin);
-  const float acc_hi = theta2 * (1.f + kMacMargin);
   // FP16 far field: group-relative frame
   const float gcx = 0.5f * (blx + bhx), gcy = 0.5f * (bly + bhy), gcz = 0.5f * (blz + bhz);
   half2_t hbx = {}, hby = {}, hbz = {}, heps2 = {};
@@ -481,10 +483,6 @@ __device__ __forceinline__ void walk_group64(
       const float ey = fmaxf(fmaxf(bly - src.y, src.y - bhy), 0.f);
       const float ez = fmaxf(fmaxf(blz - src.z, src.z - bhz), 0.f);
       const float dmin2 = ex * ex + ey * ey + ez * ez;
-      const float fx = fmaxf(fabsf(src.x - blx), fabsf(src.x - bhx));
-      const float fy = fmaxf(fabsf(src.y - bly), fabsf(src.y - bhy));
-      const float fz = fmaxf(fabsf(src.z - blz), fabsf(src.z - bhz));
-      const float dmax2 = fx * fx + fy * fy + fz * fz;
       const bool sure_accept = s2 < acc_lo * dmin2;
       // all representative tests are evaluated (no short-circuit branches):
       // open if one of them has theta2*d2 <= s2, i.e. the smallest d2 decides
@@ -492,7 +490,7 @@ __device__ __forceinline__ void walk_group64(
 #pragma unroll
       for (int r = 1; r < kReps; ++r)
         rmin = fminf(rmin, dist2(src.x - rx[r], src.y - ry[r], src.z - rz[r]));
-      sure_open = (s2 > acc_hi * dmax2) | !(s2 < theta2 * rmin);
+      sure_open = !(s2 < theta2 * rmin);
       ambiguous = !sure_accept & !sure_open;
       if (HALF)
         safe16 = (src.w <= kHalfMaxAccel * (dmin2 + eps2)) &
@@ -776,7 +774,6 @@ __device__ __forceinline__ void walk_split64(
     }
   }
   const float acc_lo = theta2 * (1.f - kMacMargin);
-  const float acc_hi = theta2 * (1.f + kMacMargin);
 
   float ax = 0.f, ay = 0.f, az = 0.f, cx = 0.f, cy = 0.f, cz = 0.f, pot = 0.f;
   int sp = 1;
@@ -833,16 +830,12 @@ __device__ __forceinline__ void walk_split64(
         const float ey = fmaxf(fmaxf(bly[s] - src.y, src.y - bhy[s]), 0.f);
         const float ez = fmaxf(fmaxf(blz[s] - src.z, src.z - bhz[s]), 0.f);
         const float dmin2 = ex * ex + ey * ey + ez * ez;
-        const float fx = fmaxf(fabsf(src.x - blx[s]), fabsf(src.x - bhx[s]));
-        const float fy = fmaxf(fabsf(src.y - bly[s]), fabsf(src.y - bhy[s]));
-        const float fz = fmaxf(fabsf(src.z - blz[s]), fabsf(src.z - bhz[s]));
-        const float dmax2 = fx * fx + fy * fy + fz * fz;
         float rmin = dist2(src.x - rx[s][0], src.y - ry[s][0], src.z - rz[s][0]);
 #pragma unroll
         for (int r = 1; r < R; ++r)
           rmin = fminf(rmin, dist2(src.x - rx[s][r], src.y - ry[s][r], src.z - rz[s][r]));
         const bool sure_accept = s2 < acc_lo * dmin2;
-        const bool sure_open = (s2 > acc_hi * dmax2) | !(s2 < theta2 * rmin);
+        const bool sure_open = !(s2 < theta2 * rmin);
         open_bits |= (unsigned)sure_open << s;
         amb_bits |= (unsigned)(!sure_accept & !sure_open) << s;
       }
