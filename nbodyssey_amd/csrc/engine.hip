@@ -687,7 +687,7 @@ class Engine {
     bh_partial.alloc(2 * kBboxBlocks);
     quant.alloc(1, true);
     state.alloc(1, true);
-    max_tiles = (n + kSortTile - 1) / kSortTile;
+    max_tiles = sort_max_tiles(n);
     lookback.alloc((size_t)kMaxPasses * max_tiles * 256, true);
     // the sort state is reset by each step's Karras kernel for the next step;
     // start from a reset state on this stream
@@ -1336,7 +1336,7 @@ int nb_sort_pairs(const uint64_t* keys, int n, int key_bits, uint64_t* keys_sort
     if (key_bits != 30 && key_bits != 63) throw nb::Error(nb::kErrInvalid, "key_bits must be 30 or 63");
     nb::check_device();
     const int passes = nb::sort_passes(key_bits);
-    const int tiles = (n + nb::kSortTile - 1) / nb::kSortTile;
+    const int tiles = nb::sort_max_tiles(n);
     nb::DevBuf<uint64_t> ka, kb;
     nb::DevBuf<unsigned> va, vb;
     nb::DevBuf<nb::SortState> ss;

@@ -870,11 +870,8 @@ template <typename K>
 void sort_typed(hipStream_t s, K* ka, K* kb, unsigned* va, unsigned* vb, int n, SortState* ss,
                 unsigned long long* lookback, int max_tiles, bool first_identity,
                 const float4* gsrc, float4* gdst, int passes) {
-  // large inputs sort faster with 6144-key tiles (fewer tiles and look-back
-  // steps; measured 4.2 vs 5.0 ms for 8 passes over 33.5M keys), smaller ones
-  // with 4096-key tiles (more workgroups to fill the chip)
-  const bool big = n >= (1 << 23);
-  const int tile_keys = kBlock * (big ? kSortItemsBig : kSortItems);
+  const int items = sort_items(n);
+  const int tile_keys = kBlock * items;
   int tiles = (n + tile_keys - 1) / tile_keys;
   if (tiles > max_tiles) throw Error(kErrState, "sort look-back buffer too small");
   for (int p = 0; p < passes; ++p) {
@@ -887,16 +884,21 @@ void sort_typed(hipStream_t s, K* ka, K* kb, unsigned* va, unsigned* vb, int n, 
 #define NB_SWEEP(F, G, I)                                                                  \
   onesweep_kernel<K, F, G, I><<<tiles, kBlock, 0, s>>>(kin, vin, kout, vout, n, p, ss, lookback, \
                                                        max_tiles, gsrc, gdst)
-    if (big) {
+    if (items == kSortItemsBig) {
       if (first && gather) NB_SWEEP(true, true, kSortItemsBig);
       else if (first) NB_SWEEP(true, false, kSortItemsBig);
       else if (gather) NB_SWEEP(false, true, kSortItemsBig);
       else NB_SWEEP(false, false, kSortItemsBig);
-    } else {
+    } else if (items == kSortItems) {
       if (first && gather) NB_SWEEP(true, true, kSortItems);
       else if (first) NB_SWEEP(true, false, kSortItems);
       else if (gather) NB_SWEEP(false, true, kSortItems);
       else NB_SWEEP(false, false, kSortItems);
+    } else {
+      if (first && gather) NB_SWEEP(true, true, kSortItemsSmall);
+      else if (first) NB_SWEEP(true, false, kSortItemsSmall);
+      else if (gather) NB_SWEEP(false, true, kSortItemsSmall);
+      else NB_SWEEP(false, false, kSortItemsSmall);
     }
 #undef NB_SWEEP
     NB_LAUNCH_CHECK();

@@ -48,9 +48,25 @@ enum Status {
 constexpr int kBlock = 256;          // kForceBlock (forces.cuh:25)
 constexpr int kStackDepth = 128;     // kBhStack (forces_bh.cu:43)
 constexpr int kBboxBlocks = 1024;    // max partial boxes written by the bbox / kick-drift-bbox pass
-constexpr int kSortItems = 16;       // keys per thread per onesweep tile
-constexpr int kSortItemsBig = 24;    // the same for n >= 2^23
-constexpr int kSortTile = kBlock * kSortItems;   // smallest tile (sizes the look-back)
+// Keys per thread of a onesweep tile, by key count: more, smaller tiles keep
+// more workgroups per CU below 2^19 keys (100K 63-bit keys: 114 -> 94 us for
+// 8 passes), 16 per thread up to 2^23 (1M: 103 us against 122 with 8), and
+// 24 above (fewer tiles and look-back steps: 33.5M 63-bit keys 5.0 -> 4.2 ms).
+constexpr int kSortItemsSmall = 8;
+constexpr int kSortItems = 16;
+constexpr int kSortItemsBig = 24;
+inline int sort_items(int n) {
+  return n >= (1 << 23) ? kSortItemsBig : n >= (1 << 19) ? kSortItems : kSortItemsSmall;
+}
+// Look-back rows needed by any sort of at most n keys (a buffer sized for n
+// also serves smaller sorts, whose tiles may be smaller).
+inline int sort_max_tiles(int n) {
+  auto tiles = [](int m) { return (m + kBlock * sort_items(m) - 1) / (kBlock * sort_items(m)); };
+  int t = tiles(n);
+  if (n >= (1 << 19)) t = t > tiles((1 << 19) - 1) ? t : tiles((1 << 19) - 1);
+  if (n >= (1 << 23)) t = t > tiles((1 << 23) - 1) ? t : tiles((1 << 23) - 1);
+  return t;
+}
 constexpr int kMaxPasses = 8;
 
 // One internal tree node, 32 bytes so a wave-uniform walk fetches it with a
