@@ -191,12 +191,13 @@ __device__ __forceinline__ void flush_half(const half2_t* l16, int npairs, half2
 // of its own set (the others contribute an exact zero), so every target sums
 // exactly its own interaction list, in batch order.
 __device__ __forceinline__ void flush_masked(const float4* list, const unsigned* mw, int cnt,
-                                             int sh, float bx, float by, float bz, float eps2,
+                                             int bit, float bx, float by, float bz, float eps2,
                                              float& ax, float& ay, float& az, int& taken) {
 #pragma unroll 8
   for (int k = 0; k < cnt; ++k) {
     const float4 s = list[k];
-    const int keep = (int)(mw[k] << sh) >> 31;  // all ones iff this lane is in the set
+    // all ones iff this lane is in the set (one signed bit-field extract)
+    const int keep = __builtin_amdgcn_sbfe((int)mw[k], (unsigned)bit, 1u);
     const float dx = s.x - bx, dy = s.y - by, dz = s.z - bz;
     const float inv = rsq(fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2))));
     const float f = __int_as_float(__float_as_int(s.w * inv * inv * inv) & keep);
@@ -225,9 +226,9 @@ __device__ __forceinline__ void walk_target64(
   const float bhx = wave_max(b.x), bhy = wave_max(b.y), bhz = wave_max(b.z);
   const float acc_lo = theta2 * (1.f - kMacMargin);
   const float acc_hi = theta2 * (1.f + kMacMargin);
-  // this lane's word of a 64-bit set, shifted so that its bit is the sign bit
+  // this lane's word of a 64-bit set and its bit in it
   const unsigned* mw = lane < 32 ? mlo : mhi;
-  const int sh = 31 - (lane & 31);
+  const int bit = lane & 31;
 
   float ax = 0.f, ay = 0.f, az = 0.f;
   int taken = 0, tested = 0;
@@ -322,7 +323,7 @@ __device__ __forceinline__ void walk_target64(
         mhi[64 + r] = (unsigned)(m >> 32);
       }
       const int nn = __popcll(nm);
-      for (int r = 0; r < nn; ++r) tested += (int)((mw[64 + r] << sh) >> 31);
+      for (int r = 0; r < nn; ++r) tested += (int)((mw[64 + r] >> bit) & 1u);
     }
     // push the children of nodes some lane opened, as a pair with that set
     const unsigned long long pm = __builtin_amdgcn_ballot_w64(om != 0ull);
@@ -348,7 +349,7 @@ __device__ __forceinline__ void walk_target64(
     }
     nlist += __popcll(apm);
     if (nlist >= 64) {
-      flush_masked(list, mw, 64, sh, b.x, b.y, b.z, eps2, ax, ay, az, taken);
+      flush_masked(list, mw, 64, bit, b.x, b.y, b.z, eps2, ax, ay, az, taken);
       nlist -= 64;
       if (lane < nlist) {
         list[lane] = list[64 + lane];
@@ -357,7 +358,7 @@ __device__ __forceinline__ void walk_target64(
       }
     }
   }
-  if (nlist > 0) flush_masked(list, mw, nlist, sh, b.x, b.y, b.z, eps2, ax, ay, az, taken);
+  if (nlist > 0) flush_masked(list, mw, nlist, bit, b.x, b.y, b.z, eps2, ax, ay, az, taken);
   if (COUNT && live) {
     counts[t] = taken;
     opens[t] = tested;
