@@ -494,8 +494,8 @@ __device__ __forceinline__ void walk_group64(
       sure_open = !(s2 < theta2 * rmin);
       ambiguous = !sure_accept & !sure_open;
       if (HALF)
-        safe16 = (src.w <= kHalfMaxAccel * (dmin2 + eps2)) &
-                 (fabsf(src.x - gcx) < kHalfMaxCoord) & (fabsf(src.y - gcy) < kHalfMaxCoord) &
+        safe16 = (src.w <= kHalfMaxAccel * (dmin2 + eps2)) &&
+                 (fabsf(src.x - gcx) < kHalfMaxCoord) && (fabsf(src.y - gcy) < kHalfMaxCoord) &&
                  (fabsf(src.z - gcz) < kHalfMaxCoord);
     }
     unsigned long long open_mask = __builtin_amdgcn_ballot_w64(sure_open);
