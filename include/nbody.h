@@ -182,12 +182,14 @@ int nb_total_energy_tree(nb_engine* h, float theta, double out[3]);
  * configurations for `steps` leapfrog steps each on the current state, and
  * keeps the fastest whose median force error against FP64 direct summation
  * (256 fixed targets) is <= max_median_err (if none is, the most accurate).
- * Candidates: Barnes-Hut -- walk group width 64/32/16 x key bits 30/63 x
- * graph replay off/on; all-pairs -- source split x graph; ensemble --
- * persistent-walk workgroups x graph.  The particle state is saved and
- * restored (tuning does not advance the simulation); the choice stays set
- * on the engine.  log (may be NULL) receives one line per candidate, the
- * chosen one first. */
+ * Candidates: Barnes-Hut group walk -- group width 64/32/16 x key bits
+ * 30/63 x graph replay off/on; Barnes-Hut per-target walk (traverse THREAD,
+ * whose approximation is kept) -- key bits x graph; all-pairs -- source
+ * split x graph; ensemble -- persistent-walk workgroups x graph.  The
+ * particle state is saved and restored (tuning does not advance the
+ * simulation), also when a candidate fails, in which case the engine keeps
+ * the caller's parameters; otherwise the choice stays set on the engine.
+ * log (may be NULL) receives one line per candidate, the chosen one first. */
 typedef struct nb_tune_result {
   int group_width;
   int key_bits;

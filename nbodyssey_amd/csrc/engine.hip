@@ -76,10 +76,9 @@ void warm_kernels() {
   done = true;
 }
 
-struct Params {
-  float dt = 0.01f, eps = 0.05f, G = 1.0f, theta = 0.5f;
-  int force = 0, traverse = 0, key_bits = 63, group_width = 32, precision = 0;
-};
+}  // namespace
+
+void set_last_error(const char* msg) { g_last_error = msg; }
 
 void validate(const Params& p) {
   if (p.force != 0 && p.force != 1) throw Error(kErrInvalid, "force must be 0 (naive) or 1 (bh)");
@@ -93,12 +92,103 @@ void validate(const Params& p) {
     throw Error(kErrInvalid,
                 "precision 2 (FP16 far field) applies to the Barnes-Hut wave64 walk "
                 "(force=bh, traverse=warp, group_width=64)");
-  if (p.precision == 2 && !(p.eps >= 0.008f))
-    throw Error(kErrInvalid, "precision 2 (FP16 far field) needs epsilon >= 0.008 "
-                             "(eps^2 must be a normal FP16 number)");
+  if (p.precision == 2 && !(p.eps >= 0.008f && p.eps <= 64.f))
+    throw Error(kErrInvalid, "precision 2 (FP16 far field) needs 0.008 <= epsilon <= 64 "
+                             "(eps^2 must be a normal FP16 number, and r^2 + eps^2 must stay "
+                             "inside FP16 range)");
   if (!(p.theta >= 0.f)) throw Error(kErrInvalid, "theta must be >= 0");
   if (!(p.eps >= 0.f)) throw Error(kErrInvalid, "epsilon must be >= 0");
 }
+
+Params from_c(const nb_params* c) {
+  Params p;
+  p.dt = c->dt;
+  p.eps = c->epsilon;
+  p.G = c->G;
+  p.theta = c->theta;
+  p.force = c->force;
+  p.traverse = c->traverse;
+  p.key_bits = c->key_bits;
+  p.group_width = c->group_width;
+  p.precision = c->precision;
+  return p;
+}
+
+BhScratch carve_bh_scratch(void* base, int n, size_t* bytes) {
+  // 256-byte aligned pieces of one block, in the order of the fields
+  size_t off = 0;
+  auto take = [&](size_t b) {
+    const size_t at = off;
+    off += (b + 255) & ~size_t(255);
+    return base ? static_cast<char*>(base) + at : nullptr;
+  };
+  const size_t c = (size_t)(n > 1 ? n : 1);
+  BhScratch s{};
+  // fixed-size bookkeeping first, then the per-particle arrays
+  s.state = reinterpret_cast<SortState*>(take(sizeof(SortState)));
+  s.quant = reinterpret_cast<QuantParams*>(take(sizeof(QuantParams)));
+  s.bbox_partial = reinterpret_cast<float4*>(take(2 * (size_t)kBboxBlocks * sizeof(float4)));
+  s.max_tiles = sort_max_tiles((int)c);
+  s.lookback = reinterpret_cast<unsigned long long*>(
+      take((size_t)kMaxPasses * s.max_tiles * 256 * sizeof(unsigned long long)));
+  s.keys_a = reinterpret_cast<uint64_t*>(take(c * sizeof(uint64_t)));
+  s.keys_b = reinterpret_cast<uint64_t*>(take(c * sizeof(uint64_t)));
+  s.vals_a = reinterpret_cast<unsigned*>(take(c * sizeof(unsigned)));
+  s.vals_b = reinterpret_cast<unsigned*>(take(c * sizeof(unsigned)));
+  s.pos_sorted = reinterpret_cast<float4*>(take(c * sizeof(float4)));
+  s.nodes = reinterpret_cast<Node*>(take(c * sizeof(Node)));
+  s.up = reinterpret_cast<int4*>(take(2 * c * sizeof(int4)));
+  s.visit = reinterpret_cast<unsigned*>(take(c * sizeof(unsigned)));
+  s.box_lo = reinterpret_cast<float4*>(take(c * sizeof(float4)));
+  s.box_hi = reinterpret_cast<float4*>(take(c * sizeof(float4)));
+  s.frontier = reinterpret_cast<int*>(take((size_t)com_scratch_ints((int)c) * sizeof(int)));
+  if (bytes) *bytes = off;
+  return s;
+}
+
+void enqueue_bh_force(hipStream_t stream, const BhScratch& b, float4* pos, float4* vel,
+                      float4* acc, int n, const Params& p, bool step, PhaseMarks* marks) {
+  const float half = 0.5f * p.dt;
+  auto mark = [&](int k) {
+    if (marks) marks->mark(k);
+  };
+  if (step)
+    launch_kick_drift(stream, pos, vel, acc, n, half, p.dt, true, b.bbox_partial);
+  else
+    launch_bbox(stream, pos, n, b.bbox_partial);
+  mark(0);
+  launch_morton_hist(stream, pos, n, b.bbox_partial, bbox_parts(n), b.quant, p.key_bits,
+                     b.keys_a, b.state);
+  mark(1);
+  launch_sort(stream, p.key_bits, b.keys_a, b.keys_b, b.vals_a, b.vals_b, n, b.state, b.lookback,
+              b.max_tiles, true, pos, b.pos_sorted);
+  mark(2);
+  launch_karras(stream, p.key_bits, b.keys_a, n, b.nodes, b.up, b.visit, b.state,
+                sort_passes(p.key_bits));
+  mark(3);
+  launch_com(stream, b.pos_sorted, n, b.nodes, b.up, b.visit, b.box_lo, b.box_hi, b.frontier,
+             &b.state->error);
+  mark(4);
+  WalkArgs w{};
+  w.pos_sorted = b.pos_sorted;
+  w.perm = b.vals_a;
+  w.n = n;
+  w.nodes = b.nodes;
+  w.theta2 = p.theta * p.theta;
+  w.eps2 = p.eps * p.eps;
+  w.G = p.G;
+  w.acc = acc;
+  w.vel = step ? vel : nullptr;
+  w.half_dt = half;
+  w.overflow = &b.state->overflow;
+  w.error = &b.state->error;
+  w.compensated = p.precision == 1;
+  w.half_far = p.precision == 2;
+  launch_walk(stream, p.traverse, p.group_width, w);
+  mark(5);
+}
+
+namespace {
 
 // Events of one timed force call, resolved lazily (no host sync per call).
 struct CallEvents {
@@ -172,40 +262,14 @@ class Engine {
     if (nsys > 0) {
       ens_force_pass(step, ce);
     } else if (bh && n >= 2) {
-      const int passes = sort_passes(p.key_bits);
-      if (step)
-        launch_kick_drift(stream, pos, vel, acc, n, half, p.dt, true, bh_partial.p);
-      else
-        launch_bbox(stream, pos, n, bh_partial.p);
-      mark(ce, kPhBbox + 1);
-      launch_morton_hist(stream, pos, n, bh_partial.p, bbox_parts(n), quant.p, p.key_bits,
-                         keys_a.p, state.p);
-      mark(ce, kPhMorton + 1);
-      launch_sort(stream, p.key_bits, keys_a.p, keys_b.p, vals_a.p, vals_b.p, n, state.p,
-                  lookback.p, max_tiles, true, pos, pos_sorted.p);
-      mark(ce, kPhSort + 1);
-      launch_karras(stream, p.key_bits, keys_a.p, n, nodes.p, up.p, visit.p, state.p, passes);
-      mark(ce, kPhBuild + 1);
-      launch_com(stream, pos_sorted.p, n, nodes.p, up.p, visit.p, box_lo.p, box_hi.p,
-                 frontier.p, &state.p->error);
-      mark(ce, kPhCom + 1);
-      WalkArgs w{};
-      w.pos_sorted = pos_sorted.p;
-      w.perm = vals_a.p;
-      w.n = n;
-      w.nodes = nodes.p;
-      w.theta2 = p.theta * p.theta;
-      w.eps2 = eps2;
-      w.G = p.G;
-      w.acc = acc;
-      w.vel = step ? vel : nullptr;
-      w.half_dt = half;
-      w.overflow = &state.p->overflow;
-      w.error = &state.p->error;
-      w.compensated = p.precision == 1;
-      w.half_far = p.precision == 2;
-      launch_walk(stream, p.traverse, p.group_width, w);
-      mark(ce, kPhWalk + 1);
+      struct Marks : PhaseMarks {
+        Engine* e;
+        CallEvents* ce;
+        void mark(int phase) override { e->mark(*ce, phase + 1); }
+      } marks;
+      marks.e = this;
+      marks.ce = &ce;
+      enqueue_bh_force(stream, scratch_view(), pos, vel, acc, n, p, step, &marks);
       tree_valid = true;
     } else if (bh) {
       // n < 2: zero force (forces_bh.cu:466-469)
@@ -403,7 +467,9 @@ class Engine {
     if (flag[0] || flag[1]) {
       int zero[2] = {0, 0};
       NB_HIP(hipMemcpy(&state.p->overflow, zero, 2 * sizeof(int), hipMemcpyHostToDevice));
-      if (flag[0]) throw Error(kErrOverflow, "tree walk stack overflow (depth > 128)");
+      if (flag[0])
+        throw Error(kErrOverflow, "tree walk stack overflow (the walk needed more entries than "
+                                  "its LDS stack holds)");
       throw Error(kErrState, "inconsistent sort/tree data detected on the device");
     }
   }
@@ -479,6 +545,11 @@ class Engine {
       const int full = walk_items_resident();
       for (int g : {full, full * 3 / 4, full / 2})
         for (int gr = 0; gr < 2; ++gr) cands.push_back({64, 30, gr, 0, std::max(g, 1), 0, 0});
+    } else if (p.force == 1 && p.traverse == 0) {
+      // the per-target walk keeps its own approximation: only key width and
+      // graph replay are tuned (group widths would change the walk's result)
+      for (int kb : {30, 63})
+        for (int gr = 0; gr < 2; ++gr) cands.push_back({p.group_width, kb, gr, 0, 0, 0, 0});
     } else if (p.force == 1) {
       for (int kb : {30, 63})
         for (int w : {64, 32, 16})
@@ -499,7 +570,6 @@ class Engine {
       for (Candidate& c : cands) {
         Params q = p0;
         if (nsys == 0 && q.force == 1) {
-          q.traverse = 1;
           q.group_width = c.group_width;
           q.key_bits = c.key_bits;
         }
@@ -538,9 +608,22 @@ class Engine {
         check_overflow();
       }
     } catch (...) {
+      // leave the engine as the caller had it: parameters, launch choices and
+      // the particle state (a candidate may have advanced it before failing)
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
       timing_on = timing0;
+      try {
+        (void)hipStreamSynchronize(stream);
+        set_params(p0);
+        ap_split = split0;
+        walk_grid = grid0;
+        drop_graph();
+        graph_mode = graph0;
+        restore();
+        (void)hipStreamSynchronize(stream);
+      } catch (...) {
+      }
       throw;
     }
     (void)hipEventDestroy(e0);
@@ -558,7 +641,6 @@ class Engine {
     const Candidate& c = cands[best];
     Params q = p0;
     if (nsys == 0 && q.force == 1) {
-      q.traverse = 1;
       q.group_width = c.group_width;
       q.key_bits = c.key_bits;
     }
@@ -569,9 +651,6 @@ class Engine {
     graph_mode = c.graph != 0;
     restore();
     NB_HIP(hipStreamSynchronize(stream));
-    (void)split0;
-    (void)grid0;
-    (void)graph0;
     std::swap(cands[0], cands[best]);  // the chosen one first
     return cands;
   }
@@ -669,6 +748,26 @@ class Engine {
   }
 
   // ---------------------------------------------------------------- scratch
+  BhScratch scratch_view() {
+    BhScratch b{};
+    b.keys_a = keys_a.p;
+    b.keys_b = keys_b.p;
+    b.vals_a = vals_a.p;
+    b.vals_b = vals_b.p;
+    b.pos_sorted = pos_sorted.p;
+    b.nodes = nodes.p;
+    b.up = up.p;
+    b.visit = visit.p;
+    b.box_lo = box_lo.p;
+    b.box_hi = box_hi.p;
+    b.frontier = frontier.p;
+    b.bbox_partial = bh_partial.p;
+    b.quant = quant.p;
+    b.state = state.p;
+    b.lookback = lookback.p;
+    b.max_tiles = max_tiles;
+    return b;
+  }
   void ensure_bh() {
     if (bh_cap >= n) return;
     release_bh();
@@ -809,20 +908,6 @@ class Engine {
   std::vector<hipEvent_t> free_events_;
   std::vector<CallEvents> pending_;
 };
-
-Params from_c(const nb_params* c) {
-  Params p;
-  p.dt = c->dt;
-  p.eps = c->epsilon;
-  p.G = c->G;
-  p.theta = c->theta;
-  p.force = c->force;
-  p.traverse = c->traverse;
-  p.key_bits = c->key_bits;
-  p.group_width = c->group_width;
-  p.precision = c->precision;
-  return p;
-}
 
 template <typename F>
 int guarded(F&& f) {

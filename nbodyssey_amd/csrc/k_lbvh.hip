@@ -132,6 +132,7 @@ __global__ __launch_bounds__(kBlock) void sort_init_kernel(SortState* ss, int pa
 
 constexpr unsigned long long kFlagAgg = 1ull << 32;
 constexpr unsigned long long kFlagInc = 2ull << 32;
+constexpr unsigned kLookbackSpins = 1u << 20;
 
 template <typename K, bool FIRST, bool GATHER, int ITEMS>
 __global__ __launch_bounds__(kBlock) void onesweep_kernel(
@@ -210,10 +211,19 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
   } else {
     store_agent(row + (size_t)tile * 256 + tid, tag | kFlagAgg | total);
     int k = (int)tile - 1;
-    while (true) {
+    // Every earlier tile publishes within microseconds of this one; a word
+    // that stays stale for kLookbackSpins polls (tens of milliseconds) means
+    // the sort state was not reset, so the tile reports an error and still
+    // publishes an inclusive count, and no later tile waits on it forever.
+    unsigned spins = 0;
+    while (k >= 0) {
       unsigned long long v = load_agent(row + (size_t)k * 256 + tid);
       unsigned long long flag = v & (3ull << 32);
       if ((unsigned)(v >> 34) != epoch || flag == 0) {
+        if (++spins > kLookbackSpins) {
+          ss->error = 1;
+          break;
+        }
         __builtin_amdgcn_s_sleep(1);
         continue;
       }

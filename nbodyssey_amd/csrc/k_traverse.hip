@@ -139,13 +139,20 @@ __device__ __forceinline__ void flush_list(const float4* list, int cnt, float bx
 // safely inside FP16 range are evaluated two per lane-instruction with packed
 // FP16 arithmetic, positions relative to the group's box centre and masses
 // scaled by 512; every product is accumulated in FP32 (mixed FMA) and scaled
-// back once.  A cell goes to this list only if |cell - centre| < 16384 per
-// axis and m / (dmin^2 + eps^2) <= 32 (dmin = distance to the group box), so
-// m * 512 * inv^2 <= 16384 < 65504 for every lane.
+// back once.  A cell goes to this list only if, with h the group box's
+// half-extent on each axis,
+//   |cell - centre| + h < 128 per axis: every lane's |d| < 128 per axis, so
+//       r^2 = dx^2 + dy^2 + dz^2 + eps^2 < 3 * 16384 + eps^2 < 65504;
+//   2^-14 / 512 <= m <= 64: m * 512 is a normal FP16 number below 32768;
+//   m / (dmin^2 + eps^2) <= 32 (dmin = distance to the group box): every
+//       lane has r^2 >= dmin^2 + eps^2, so m * 512 * inv^2 <= 16384.
+// Everything else (leaves, near or heavy cells) stays in the FP32 list.
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 constexpr float kHalfMassScale = 512.f;
 constexpr float kHalfMaxAccel = 32.f;
-constexpr float kHalfMaxCoord = 16384.f;
+constexpr float kHalfMaxSep = 128.f;
+constexpr float kHalfMaxMass = 64.f;
+constexpr float kHalfMinMass = 6.103515625e-05f / kHalfMassScale;  // 2^-14 / 512
 constexpr int kHalfList = 128;   // FP16 list entries per wave (64 pairs)
 
 // l16: pairs of entries, pair p = half2 (x0,x1), (y0,y1), (z0,z1), (m0,m1) at
@@ -423,6 +430,9 @@ __device__ __forceinline__ void walk_group64(
   const float acc_lo = theta2 * (1.f - kMacMargin);
   // FP16 far field: group-relative frame
   const float gcx = 0.5f * (blx + bhx), gcy = 0.5f * (bly + bhy), gcz = 0.5f * (blz + bhz);
+  // per-axis room left for a cell's offset from the centre (see kHalfMaxSep)
+  const float rmx = kHalfMaxSep - 0.5f * (bhx - blx), rmy = kHalfMaxSep - 0.5f * (bhy - bly),
+              rmz = kHalfMaxSep - 0.5f * (bhz - blz);
   half2_t hbx = {}, hby = {}, hbz = {}, heps2 = {};
   float fx = 0.f, fy = 0.f, fz = 0.f;
   int n16 = 0;
@@ -494,9 +504,9 @@ __device__ __forceinline__ void walk_group64(
       sure_open = !(s2 < theta2 * rmin);
       ambiguous = !sure_accept & !sure_open;
       if (HALF)
-        safe16 = (src.w <= kHalfMaxAccel * (dmin2 + eps2)) &&
-                 (fabsf(src.x - gcx) < kHalfMaxCoord) && (fabsf(src.y - gcy) < kHalfMaxCoord) &&
-                 (fabsf(src.z - gcz) < kHalfMaxCoord);
+        safe16 = (src.w <= kHalfMaxAccel * (dmin2 + eps2)) && (src.w <= kHalfMaxMass) &&
+                 (src.w >= kHalfMinMass) && (fabsf(src.x - gcx) < rmx) &&
+                 (fabsf(src.y - gcy) < rmy) && (fabsf(src.z - gcz) < rmz);
     }
     unsigned long long open_mask = __builtin_amdgcn_ballot_w64(sure_open);
     unsigned long long amb = __builtin_amdgcn_ballot_w64(ambiguous);

@@ -100,6 +100,16 @@ struct SortState {
   int pad[3];
 };
 
+// Run parameters (SimParams of simulation.hpp:8-15 plus the additive knobs of
+// nb_params); validate() throws kErrInvalid on an unsupported combination.
+struct Params {
+  float dt = 0.01f, eps = 0.05f, G = 1.0f, theta = 0.5f;
+  int force = 0, traverse = 0, key_bits = 63, group_width = 32, precision = 0;
+};
+void validate(const Params& p);
+// Message returned by nb_last_error() on this thread.
+void set_last_error(const char* msg);
+
 // ---------------------------------------------------------------- launchers
 // Every launcher enqueues on `s` and checks the launch.
 
@@ -226,4 +236,50 @@ void launch_ens_energy(hipStream_t s, const float4* pos, const float4* vel, cons
                        const int2* tiles, int ntiles, const int* tile_off, int nsys, float eps2,
                        float G, double* partial, double* out);
 
+// ---------------------------------------------------------------- BH pass
+// Device buffers of one Barnes-Hut force evaluation over n particles.
+struct BhScratch {
+  uint64_t* keys_a;     // n keys (u32 words for 30-bit keys), sort double buffer
+  uint64_t* keys_b;
+  unsigned* vals_a;     // n original indices; after the sort: perm
+  unsigned* vals_b;
+  float4* pos_sorted;   // n
+  Node* nodes;          // n - 1
+  int4* up;             // 2n - 1
+  unsigned* visit;      // n - 1
+  float4* box_lo;       // n - 1
+  float4* box_hi;       // n - 1
+  int* frontier;        // com_scratch_ints(n)
+  float4* bbox_partial; // 2 * kBboxBlocks
+  QuantParams* quant;
+  SortState* state;     // reset by launch_sort_init before the first pass
+  unsigned long long* lookback;  // kMaxPasses * max_tiles * 256
+  int max_tiles;
+};
+// The buffers for n particles carved out of one block at `base` in 256-byte
+// aligned pieces, the fixed-size ones (state, quant, bbox_partial, lookback)
+// first and keys_a the first per-particle piece; *bytes receives the block
+// size (base may be null to size it).  A carve for n also serves any smaller
+// particle count.
+BhScratch carve_bh_scratch(void* base, int n, size_t* bytes);
+
+// Phase hook: mark(k) is called after phase k (bbox, morton, sort, build, com,
+// traverse) has been enqueued.
+struct PhaseMarks {
+  virtual void mark(int phase) = 0;
+};
+// One Barnes-Hut force evaluation (n >= 2) on `pos` into `acc` (original
+// order), enqueued on `stream` (forces_bh.cu:463-527).  step: the first kick
+// and the drift run first (fused with the bbox) and the closing kick rides in
+// the walk's epilogue on `vel`.  The sort state must have been reset (by
+// launch_sort_init, or by the previous pass's Karras kernel).
+void enqueue_bh_force(hipStream_t stream, const BhScratch& b, float4* pos, float4* vel,
+                      float4* acc, int n, const Params& p, bool step, PhaseMarks* marks);
+
+}  // namespace nb
+
+// C ABI parameter block (include/nbody.h).
+struct nb_params;
+namespace nb {
+Params from_c(const ::nb_params* c);
 }  // namespace nb

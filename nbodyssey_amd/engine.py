@@ -132,7 +132,7 @@ class Simulation:
                                   buf, len(buf)))
         choice = {k: getattr(r, k) for k, _ in _lib.NbTuneResult._fields_}
         if self.params.get("force") in ("bh", _lib.FORCE_BH) and not hasattr(self, "nsys"):
-            self.params.update(traverse="warp", group_width=r.group_width, key_bits=r.key_bits)
+            self.params.update(group_width=r.group_width, key_bits=r.key_bits)
         return choice, buf.value.decode().splitlines()
 
     def total_energy_tree(self, theta=0.3):

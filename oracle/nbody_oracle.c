@@ -31,8 +31,12 @@
  *   orc_traverse_*            : theta=0 equals FP64 all-pairs to 1e-12 in the
  *       FP64 build of the walk and touches exactly n leaves
  *       (lbvh_check.py:250-264); theta=0.5 median error < 0.05 and
- *       interactions < n/2 (lbvh_check.py:266-279); W=32 group walk equals a
- *       numpy restatement of lbvh_check.py:180-209 on tiny inputs.
+ *       interactions < n/2 (lbvh_check.py:266-279); at theta 0.3/0.5/0.8 the
+ *       per-target and the 1/16/32/64-lane group walks' interaction and
+ *       opening counts equal a Python restatement of lbvh_check.py:151-209
+ *       (traverse, traverse_warp) on <= 3,024 points, walks with an opening
+ *       test within 1e-6 of its threshold excepted
+ *       (tests/test_walk_restatement.py).
  *   orc_allpairs_*            : two-body and three-body closed forms, Newton's
  *       third law (sum of m*a == 0), FP64 vs FP32 agreement.
  *   orc_energy_f64            : closed-form two-body energy; the committed

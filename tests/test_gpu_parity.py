@@ -577,6 +577,35 @@ def test_far16_parameter_checks():
                           far16=True)
 
 
+def _blob(n, centre, radius, mass, seed):
+    rng = np.random.default_rng(seed)
+    p = rng.normal(0.0, radius, size=(n, 3)) + np.asarray(centre, np.float64)
+    return np.column_stack([p, np.full(n, mass / n)]).astype(np.float32)
+
+
+@pytest.mark.parametrize("case", ["far300", "heavy200"])
+def test_far16_range_guard(case):
+    """Cells outside FP16's range stay in FP32.  far300: a light blob whose
+    pull comes mostly from a cluster 300 away (|d|^2 = 9e4 > 65504 would
+    overflow in FP16 and drop the pull); heavy200: cells of mass 100-200
+    10 away (m * 512 > 65504 would overflow to inf).  The FP16-far forces
+    stay finite and within FP16 rounding of the FP32 walk's."""
+    if case == "far300":
+        pos = np.concatenate([_blob(4096, (0, 0, 0), 0.5, 1e-5, 1),
+                              _blob(4096, (300, 0, 0), 0.5, 1.0, 2)])
+    else:
+        pos = np.concatenate([_blob(4096, (0, 0, 0), 0.5, 1.0, 3),
+                              _blob(4096, (0, 10, 0), 0.5, 200.0, 4)])
+    kw = dict(eps=0.05, theta=0.5, force="bh", traverse="warp", group_width=64, key_bits=30)
+    with engine.Simulation(pos, **kw) as a, engine.Simulation(pos, far16=True, **kw) as b:
+        a32 = a.compute_forces()
+        a16 = b.compute_forces()
+    assert np.all(np.isfinite(a16)), case
+    d = rel_err(a16, a32)
+    print(f"far16 guard {case}: median {np.median(d):.2e} max {d.max():.2e}", flush=True)
+    assert np.median(d) < 1e-3 and d.max() < 1e-2, (np.median(d), d.max())
+
+
 def test_galaxy_sim_tree_energy_and_far16(tmp_path):
     """Additive CLI flags: --energy-theta logs the tree-potential energy (its
     potential within 1e-3 of the exact log's -- the theta 0.3 monopole error
