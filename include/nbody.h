@@ -245,6 +245,27 @@ int nb_ensemble_roots(nb_engine* h, int32_t* roots);
  * ensemble returns the sums of these (no energy between systems). */
 int nb_system_energy(nb_engine* h, double* out);
 
+/* ------------------------------------------------------- no allocation
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc -- the force evaluation of nb_compute_forces on
+ * caller-owned device arrays, with no device allocation or free:
+ *   d_pos4  n x (x, y, z, m) float4 in device memory of the current device;
+ *   d_acc4  n x float4 device output (ax, ay, az, 0), original order;
+ *   params  as for nb_set_params (force NAIVE or BH, any walk and precision);
+ *   stream  a hipStream_t (NULL: the null stream); the work is enqueued on it
+ *           and the call returns without waiting.
+ * The result is bitwise that of nb_compute_forces with the same params and
+ * positions.  Every scratch buffer lives in one static device array of
+ * 12 MiB, sized for n <= NB_NOALLOC_MAX_N particles (larger n is rejected
+ * with NB_ERR_INVALID), which each GPU the library runs this on holds until
+ * the process ends.  All calls on one GPU share that array: calls that
+ * overlap on one GPU (different streams or threads) race.  The walk's
+ * consistency flags are not read back (that would need a host wait). */
+#define NB_NOALLOC_MAX_N 65536
+int nb_compute_forces_noalloc(const float* d_pos4, int n, const nb_params* params, float* d_acc4,
+                              void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -107,7 +107,10 @@ SIGNATURES = {
     "nb_direct_f64": (_i, [_vp, _vp, _i, _vp]),
     "nb_total_energy_tree": (_i, [_vp, ctypes.c_float, _vp]),
     "nb_autotune": (_i, [_vp, _i, ctypes.c_float, ctypes.POINTER(NbTuneResult), _cp, _i]),
+    "nb_compute_forces_noalloc": (_i, [_vp, _i, _pp, _vp, _vp]),
 }
+
+NOALLOC_MAX_N = 65536  # NB_NOALLOC_MAX_N
 
 _lib = None
 
