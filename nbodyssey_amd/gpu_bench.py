@@ -20,6 +20,7 @@ for the warp walk (default 32 = the reference's warp).
 import argparse
 import os
 import re
+import shutil
 import subprocess
 import sys
 import tempfile
@@ -59,6 +60,13 @@ def main(argv=None):
     a = ap.parse_args(argv)
     os.makedirs(os.path.join(a.out, "logs"), exist_ok=True)
     scratch = tempfile.mkdtemp()
+    try:
+        return session(a, scratch)
+    finally:
+        shutil.rmtree(scratch, ignore_errors=True)
+
+
+def session(a, scratch):
     warp = ["--traverse", "warp", "--group-width", a.group_width]
 
     def make_ic(n, name):

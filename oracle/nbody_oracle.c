@@ -427,10 +427,37 @@ int orc_traverse_thread(const float* pos_sorted4, int n, const int32_t* children
  * every lane takes it (or the leaf) as a source.  Per-group interaction and
  * opening-test counts (one count per popped node, not per lane).  Returns max
  * depth or -1 on overflow. */
+static int group_walk(const float* pos_sorted4, int n, const int32_t* children,
+                      const float* com4, const float* size2, float theta, float eps, float G,
+                      int width, float* acc4_sorted, double* acc3_sorted_f64,
+                      int32_t* group_inter, int32_t* group_opens);
+
 int orc_traverse_group(const float* pos_sorted4, int n, const int32_t* children,
                        const float* com4, const float* size2, float theta, float eps, float G,
                        int width, float* acc4_sorted, int32_t* group_inter,
                        int32_t* group_opens) {
+  return group_walk(pos_sorted4, n, children, com4, size2, theta, eps, G, width, acc4_sorted,
+                    NULL, group_inter, group_opens);
+}
+
+/* The same group walk -- identical FP32 open/accept decisions, so identical
+ * interaction lists -- with each lane's list summed in FP64 from the FP32
+ * source and target values (d, r^2 + eps^2 with the walk's FP32 eps^2, the
+ * force factor and the sum all in FP64).  Against it, a force module's FP32
+ * result shows its arithmetic error alone (rounding of the per-interaction
+ * terms and of their accumulation), separated from the multipole error of
+ * the list itself.  acc3_sorted is 3 doubles per sorted target. */
+int orc_traverse_group_f64sum(const float* pos_sorted4, int n, const int32_t* children,
+                              const float* com4, const float* size2, float theta, float eps,
+                              double G, int width, double* acc3_sorted, int32_t* group_inter) {
+  return group_walk(pos_sorted4, n, children, com4, size2, theta, eps, (float)G, width, NULL,
+                    acc3_sorted, group_inter, NULL);
+}
+
+static int group_walk(const float* pos_sorted4, int n, const int32_t* children,
+                      const float* com4, const float* size2, float theta, float eps, float G,
+                      int width, float* acc4_sorted, double* acc3_sorted_f64,
+                      int32_t* group_inter, int32_t* group_opens) {
   float theta2 = theta * theta, eps2 = eps * eps;
   int ngroups = (n + width - 1) / width;
   int worst = 0, overflow = 0;
@@ -439,6 +466,7 @@ int orc_traverse_group(const float* pos_sorted4, int n, const int32_t* children,
     int base = g * width;
     int nl = n - base < width ? n - base : width;
     float* a = (float*)calloc((size_t)nl * 3, sizeof(float));
+    double* a64 = (double*)calloc((size_t)nl * 3, sizeof(double));
     int st[ORC_STACK];
     int sp = 0, ni = 0, no = 0;
     st[sp++] = 0;
@@ -467,7 +495,16 @@ int orc_traverse_group(const float* pos_sorted4, int n, const int32_t* children,
       }
       for (int l = 0; l < nl; ++l) {
         const float* b = pos_sorted4 + 4 * (size_t)(base + l);
-        orc_interact(b[0], b[1], b[2], src, eps2, a + 3 * l);
+        if (acc4_sorted) orc_interact(b[0], b[1], b[2], src, eps2, a + 3 * l);
+        if (acc3_sorted_f64) {
+          const double dx = (double)src[0] - b[0], dy = (double)src[1] - b[1],
+                       dz = (double)src[2] - b[2];
+          const double r2 = dx * dx + dy * dy + dz * dz + (double)eps2;
+          const double f = (double)src[3] / (r2 * sqrt(r2));
+          a64[3 * l] += dx * f;
+          a64[3 * l + 1] += dy * f;
+          a64[3 * l + 2] += dz * f;
+        }
       }
       ++ni;
     }
@@ -478,9 +515,13 @@ int orc_traverse_group(const float* pos_sorted4, int n, const int32_t* children,
         acc4_sorted[4 * (size_t)(base + l) + 2] = G * a[3 * l + 2];
         acc4_sorted[4 * (size_t)(base + l) + 3] = 0.f;
       }
+    if (acc3_sorted_f64)
+      for (int l = 0; l < nl * 3; ++l)
+        acc3_sorted_f64[3 * (size_t)base + l] = (double)G * a64[l];
     if (group_inter) group_inter[g] = ni;
     if (group_opens) group_opens[g] = no;
     free(a);
+    free(a64);
   }
   return overflow ? -1 : worst;
 }

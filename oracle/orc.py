@@ -48,6 +48,8 @@ def lib():
                                           _vp, _vp, _vp]
         L.orc_traverse_group.argtypes = [_f32p, _i, _i32p, _f32p, _f32p, _f, _f, _f, _i,
                                          _vp, _vp, _vp]
+        L.orc_traverse_group_f64sum.argtypes = [_f32p, _i, _i32p, _f32p, _f32p, _f, _f, _d, _i,
+                                                _vp, _vp]
         L.orc_traverse_thread_f64.argtypes = [_f64p, _i, _i32p, _f64p, _f64p, _d, _d, _i, _i,
                                               _f64p, _vp]
         L.orc_energy_f64.argtypes = [_f32p, _f32p, _i, _d, _d, _f64p]
@@ -209,6 +211,21 @@ class Tree:
         acc = np.zeros_like(acc_s)
         acc[self.perm] = acc_s
         return acc, acc_s, gi, go, depth
+
+    def traverse_group_f64sum(self, theta, eps, width, G=1.0):
+        """The group walk's interaction lists (its FP32 open/accept decisions)
+        summed in FP64: (acc (n, 3) in original order, per-group counts)."""
+        acc_s = np.zeros((self.n, 3))
+        gi = np.zeros((self.n + width - 1) // width, np.int32)
+        c4, s2 = self._nodes()
+        depth = lib().orc_traverse_group_f64sum(self.pos_sorted, self.n, self._ch(), c4, s2,
+                                                theta, eps, G, width, acc_s.ctypes.data,
+                                                gi.ctypes.data)
+        if depth < 0:
+            raise RuntimeError("oracle traversal stack overflow")
+        acc = np.zeros_like(acc_s)
+        acc[self.perm] = acc_s
+        return acc, gi
 
 
 def traverse_thread_f64(pos_sorted4_f64, children, com4d, size2d, theta, eps, t0, t1):

@@ -198,7 +198,7 @@ def test_allpairs_vs_oracle_12k():
           f"compensated vs f64 med {np.median(ec):.2e}", flush=True)
     assert np.median(e32) < 1e-5 and e32.max() < 1e-3
     assert np.median(e64) < 1e-5 and e64.max() < 1e-3
-    assert np.median(ec) <= np.median(e64) * 1.5
+    assert np.median(ec) <= np.median(e64)
 
 
 @pytest.mark.parametrize("n", [1, 2, 5, 257, 12000])

@@ -150,6 +150,19 @@ def test_gpu_bench_covers_reference_sweep():
     assert ours == ref
 
 
+def test_gpu_bench_scrapes_like_the_reference():
+    """ms/call is the next-to-last token of the force-pipeline line
+    (gpu_bench.sh:31, awk '{print $(NF-1)}') and the gate reads the mean of
+    the --compare-forces report (gpu_bench.sh:41-44)."""
+    from nbodyssey_amd import gpu_bench
+    out = ("n=4096 dt=0.01 eps=0.05 steps=10 force=naive\n"
+           "force pipeline: 11 calls, 0.5 ms total, 0.043 ms/call\n")
+    assert gpu_bench.ms_per_call(out) == 0.043
+    gate = ("naive vs barnes-hut, theta=0, traverse=warp, n=12000\n"
+            "relative force error: max 2.801e-04, mean 1.694e-05\n")
+    assert gpu_bench.gate_mean(gate) == 1.694e-05
+
+
 # ------------------------------------------------------------- galaxy_sim CLI
 
 def run_cli(*args):

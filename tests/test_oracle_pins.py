@@ -267,6 +267,29 @@ def test_warp_walk_lbvh_check_bands():
     assert np.median(e5) < 0.05
 
 
+def test_group_walk_fp64_sum():
+    """orc_traverse_group_f64sum: the group walk's lists summed in FP64.  At
+    theta = 0 (every leaf; eps = 1/16, so the walk's FP32 eps^2 is exact) it
+    is the FP64 direct sum to summation-order round-off; at theta 0.5 it
+    keeps the FP32 walk's decisions (same counts) and the FP32 walk sits
+    within FP32 round-off of it."""
+    pos = ic.two_disk_encounter(3000, seed=8)[0]
+    t = orc.Tree(pos, 30)
+    a0, g0 = t.traverse_group_f64sum(0.0, 0.0625, 32)
+    ref = orc.allpairs_f64(pos, 0.0625)
+    e0 = np.linalg.norm(a0 - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    assert e0.max() < 1e-12 and np.all(g0 == len(pos))
+    for width in (16, 64):
+        a5, g5 = t.traverse_group_f64sum(0.5, 0.05, width)
+        a32, _, gi, _, _ = t.traverse_group(0.5, 0.05, width)
+        assert np.array_equal(g5, gi)
+        e = np.linalg.norm(a32[:, :3] - a5, axis=1) / np.linalg.norm(a5, axis=1)
+        assert np.median(e) < 2e-6 and e.max() < 1e-4
+        # and it is not the direct sum: the multipole error is still in it
+        e_ref = np.linalg.norm(a5 - orc.allpairs_f64(pos, 0.05), axis=1)
+        assert np.median(e_ref / np.linalg.norm(a5, axis=1)) > 1e-4
+
+
 # ---------------------------------------------------------- force & energy
 
 def test_two_body_closed_form():
