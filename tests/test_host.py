@@ -178,6 +178,12 @@ def run_cli(*args):
     (["--ic", "/nonexistent/ic.bin"], "cannot open IC file: /nonexistent/ic.bin"),
     (["--ic", "x", "--steps", "ten"], "bad value for --steps: ten"),
     (["--ic", "x", "--keys", "32"], "--keys must be 30 or 63, got: 32"),
+    (["--ic", "x", "--group-width", "48"], "--group-width must be 16, 32 or 64, got: 48"),
+    (["--ic", "x", "--steps", "10abc"], "bad value for --steps: 10abc"),
+    (["--ic", "x", "--dt", " 0.1"], "bad value for --dt:  0.1"),
+    (["--ic", "x", "--dump-every", "0"], "--dump-every must be positive"),
+    (["--ic", "x", "--energy-theta", "0.6"], "--energy-theta must be in [0, 0.577)"),
+    (["--ic", "x", "--theta"], "missing value for --theta"),
 ])
 def test_cli_errors(args, msg):
     assert os.path.exists(GALAXY_SIM)
