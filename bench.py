@@ -15,9 +15,16 @@ Prints one JSON line (the metric is particle-steps/s) with:
   accuracy      median / p99 relative force error of the final state's
                 accelerations against FP64 direct summation on the GPU
                 (nb_direct_f64) for sampled targets;
+  energy        relative FP64 total-energy drift over the timed steps;
+  allpairs      one tiled all-pairs launch on the initial 1M state:
+                Ginteractions/s and its fraction of the FP32 peak;
+  ensemble      256 parabolic two-disk encounters batched in one engine
+                against the same systems one at a time: both rates, their
+                ratio, whether the forces are bitwise equal, force error;
   cpu_baseline  the reference CPU algorithm (NumPy chunked float32 all-pairs
                 + KDK, scripts/reference_nbody.py:45-61,130-136 restated)
-                timed on this host on a bounded sample and extrapolated.
+                timed on this host's cores: the 12,000-particle run and one
+                65,536-particle step extrapolated to 1M.
 """
 import argparse
 import json
@@ -45,25 +52,63 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(n_target, sample_n, eps, dt):
-    """reference_nbody.py's algorithm (NumPy, float32, chunk 2048) on one step
-    of a sample_n-particle two-disk system; extrapolated to n_target as
-    O(n^2) all-pairs (the reference's CPU path has no tree code at scale)."""
+def cpu_baseline(n_target, sample_n=65536, c1_budget_s=15.0, workers=None):
+    """The reference's CPU algorithm (scripts/reference_nbody.py:45-61,129-136:
+    NumPy float32 chunked (b, n, 3) all-pairs + KDK, restated in
+    oracle/numpy_ref.py) on this host, over a pool of one worker process per
+    host core (up to 16), each evaluating one contiguous block of targets:
+
+      * the 12,000-particle two-disk run (eps 0.05, dt 0.01, 200 naive KDK
+        steps) timed wall-clock step by step, energy logging off, until
+        c1_budget_s has passed or all 200 steps have run; the full-run time is
+        extrapolated from the steps done;
+      * one KDK step of the two-disk encounter at sample_n particles (eps
+        0.02, dt 0.005), extrapolated to n_target as O(n^2) all-pairs (the
+        reference's CPU path has no tree code at scale).
+
+    Must run before this process touches the GPU (the pool forks)."""
+    from multiprocessing import get_context
     os.environ.setdefault("OMP_NUM_THREADS", "1")
+    os.environ.setdefault("OPENBLAS_NUM_THREADS", "1")
     from nbodyssey_amd import ic
     from oracle import numpy_ref
-    pos4, vel4 = ic.two_disk_encounter(sample_n, seed=1, eps=eps)
-    pos, mass, vel = pos4[:, :3].copy(), pos4[:, 3].copy(), vel4[:, :3].copy()
-    t0 = time.perf_counter()
-    numpy_ref.kdk_run(pos, vel, mass, eps, dt, 1)
-    sec = time.perf_counter() - t0   # a(x0) + one step = 2 force evaluations
-    per_step = sec / 2.0 * (n_target / sample_n) ** 2
-    return {"value": n_target / per_step, "unit": "particle-steps/s", "cores": 1,
+    workers = workers or max(1, min(16, os.cpu_count() or 1))
+    with get_context("fork").Pool(workers) as pool:
+        p4, v4 = ic.two_disk_encounter(12000, seed=0, eps=0.05)
+        t0 = time.perf_counter()
+        numpy_ref.kdk_run(p4[:, :3], v4[:, :3], p4[:, 3], 0.05, 0.01, 200, pool=pool,
+                          budget_s=c1_budget_s)
+        c1_s = time.perf_counter() - t0
+        c1_steps = numpy_ref.kdk_run.steps_done
+        p4, v4 = ic.two_disk_encounter(sample_n, seed=1, eps=0.02)
+        pos, mass, vel = p4[:, :3].copy(), p4[:, 3].copy(), v4[:, :3].copy()
+        eps2, dt = np.float32(0.02 * 0.02), np.float32(0.005)
+        acc = numpy_ref.accelerations(pos, mass, eps2, np.float32(1.0), pool=pool)
+        t0 = time.perf_counter()
+        vel += np.float32(0.5) * dt * acc        # one KDK step
+        pos += dt * vel
+        acc = numpy_ref.accelerations(pos, mass, eps2, np.float32(1.0), pool=pool)
+        vel += np.float32(0.5) * dt * acc
+        step_s = time.perf_counter() - t0
+    scale = (n_target / sample_n) ** 2
+    per_step = step_s * scale
+    # 200 steps are 201 force evaluations (a(x0) first); c1_steps steps took
+    # c1_steps + 1 of them
+    c1_full = c1_s * 201 / (c1_steps + 1)
+    return {"value": n_target / per_step, "unit": "particle-steps/s", "cores": workers,
             "kind": "port",
-            "sample": f"NumPy float32 chunked all-pairs KDK (reference_nbody.py algorithm), "
-                      f"{sample_n} particles, 2 force evaluations in {sec:.1f} s on 1 of "
-                      f"{os.cpu_count()} host cores ({cpu_model()}), extrapolated x"
-                      f"{(n_target / sample_n) ** 2:.0f} to {n_target} particles (O(n^2))"}
+            "sample": f"NumPy float32 chunked all-pairs KDK (reference_nbody.py algorithm) on "
+                      f"{workers} worker processes of {os.cpu_count()} host cores "
+                      f"({cpu_model()}): one {sample_n}-particle step in {step_s:.2f} s, "
+                      f"extrapolated x{scale:.0f} to {n_target} particles (O(n^2))",
+            "extrapolated": True,
+            "config1": {"workload": "two-disk 12000 particles, naive all-pairs KDK, eps 0.05, "
+                                    "dt 0.01, 200 steps, energy logging off",
+                        "steps_timed": c1_steps, "wall_s": c1_s,
+                        "full_run_s": c1_full,
+                        "full_run_extrapolated": c1_steps < 200,
+                        "particle_steps_per_s": 12000 * 200 / c1_full},
+            "host_cpus": os.cpu_count(), "cpu_model": cpu_model()}
 
 
 def profiled_walk_traffic(args):
@@ -131,17 +176,18 @@ def naive_main(args, sim, eps):
     sim.close()
 
 
-def ensemble_main(args):
+def ensemble_run(args, steps, serial_steps, serial_warmup):
     """Batched ensemble of independent parabolic two-disk encounters against
     the same systems run one at a time on the plain path.
 
     Batched: one engine (nb_create_ensemble), launch configuration chosen by
-    the autotuner (persistent-walk workgroups, graph replay).  Serial: one plain engine per system (Barnes-Hut, wave64 walk,
-    30-bit keys, graph mode: the fastest plain configuration), started from
-    the batched run's final state, timed per system and summed.  Both compute
-    bitwise the same forces for a given state (the forest holds each system's
-    standalone tree), so the accuracy is the same by construction.
-    """
+    the autotuner (persistent-walk workgroups x graph replay, within 1% median
+    force error).  Serial: one plain engine per system (Barnes-Hut, wave64
+    walk, 30-bit keys, graph mode: the fastest plain configuration), started
+    from the batched run's final state, timed per system and summed.  Both
+    compute bitwise the same forces for a given state (the forest holds each
+    system's standalone tree, checked for every system), so the accuracy is
+    the same by construction.  Returns a dict of the measurements."""
     from nbodyssey_amd import engine, ic
     w = ic.WORKLOADS["ensemble-256-parabolic"]
     eps, dt, theta = w["eps"], w["dt"], args.theta
@@ -157,14 +203,13 @@ def ensemble_main(args):
     ens.compute_forces()
     tuned = None
     if not args.no_graph:
-        # autotuner: persistent-walk workgroups x graph replay, within 1% median error
         choice, log = ens.autotune(steps=3, max_median_err=0.01)
         tuned = {"choice": choice, "log": log}
         print("bench: autotune " + log[0], file=sys.stderr, flush=True)
     ens.step(args.warmup)
     ens.synchronize()
-    ms_batched = ens.time_steps(args.steps)
-    print(f"bench: batched {args.steps} steps {ms_batched:.2f} ms", file=sys.stderr, flush=True)
+    ms_batched = ens.time_steps(steps)
+    print(f"bench: batched {steps} steps {ms_batched:.2f} ms", file=sys.stderr, flush=True)
     # phase times (walk kernel duration) from eager steps of the same engine
     ens.set_graph(False)
     ens.reset_force_timing()
@@ -197,53 +242,96 @@ def ensemble_main(args):
             idx = rng.choice(m, per_sys, replace=False).astype(np.int32)
             err.append(sim.force_error(idx, a_s))
             sim.set_graph(not args.no_graph)
-            sim.step(args.serial_warmup)
+            sim.step(serial_warmup)
             sim.synchronize()
-            ms_serial += sim.time_steps(args.serial_steps) / args.serial_steps
-    ms_serial_total = ms_serial * args.steps   # same step count as the batched run
-    value = n * args.steps / (ms_batched * 1e-3)
-    serial_value = n * args.steps / (ms_serial_total * 1e-3)
+            ms_serial += sim.time_steps(serial_steps) / serial_steps
+    value = n * steps / (ms_batched * 1e-3)
+    serial_value = n / (ms_serial * 1e-3)
     print(f"bench: serial {ms_serial:.3f} ms per ensemble step (sum over systems)",
           file=sys.stderr, flush=True)
     flops = 20.0 * inter_sum + 10.0 * opens_sum
-    achieved = flops / (walk_ms * 1e-3) / 1e12
+    e = np.concatenate(err)
+    return dict(
+        systems=args.systems, n=n, theta=theta, eps=eps, dt=dt, steps=steps,
+        ms_batched=ms_batched, value=value, serial_value=serial_value, ms_serial=ms_serial,
+        serial_steps=serial_steps, tuned=tuned, bitwise=not mismatched, phase=phase,
+        calls=calls, walk_ms=walk_ms, flops=flops,
+        accuracy={"median_rel_force_err_vs_fp64": float(np.median(e)),
+                  "p99_rel_force_err_vs_fp64": float(np.quantile(e, 0.99)),
+                  "targets": int(len(e)),
+                  "reference": "FP64 direct summation on the GPU (nb_direct_f64)"})
+
+
+def ensemble_summary(r, graph=True):
+    """The ensemble measurements as one JSON object."""
+    return {
+        "workload": f"{r['systems']} parabolic two-disk encounters, {r['n']} particles "
+                    f"(n_s log-uniform 2048..65536), Barnes-Hut rebuilt every step, (system, "
+                    f"30-bit) keys, theta={r['theta']}, eps={r['eps']}, dt={r['dt']}, wave64 "
+                    f"persistent walk, {'graph-captured' if graph else 'eager'} step",
+        "particle_steps_per_s": r["value"], "ms_per_step": r["ms_batched"] / r["steps"],
+        "steps": r["steps"],
+        "serial_particle_steps_per_s": r["serial_value"],
+        "serial_ms_per_ensemble_step": r["ms_serial"],
+        "serial_how": f"each system alone on the plain path (bh, wave64, 30-bit keys, "
+                      f"{'graph mode' if graph else 'eager'}), {r['serial_steps']} timed steps "
+                      f"each, per-step times summed",
+        "batched_over_serial": r["value"] / r["serial_value"],
+        "batched_forces_bitwise_equal_serial": r["bitwise"],
+        "accuracy": r["accuracy"],
+        "autotune_choice": None if r["tuned"] is None else r["tuned"]["choice"],
+    }
+
+
+def ensemble_main(args):
+    r = ensemble_run(args, args.steps, args.serial_steps, args.serial_warmup)
+    achieved = r["flops"] / (r["walk_ms"] * 1e-3) / 1e12
     names = ["bbox", "morton", "sort", "build", "com", "traverse"]
-    accuracy = None
-    if err:
-        e = np.concatenate(err)
-        accuracy = {"median_rel_force_err_vs_fp64": float(np.median(e)),
-                    "p99_rel_force_err_vs_fp64": float(np.quantile(e, 0.99)),
-                    "targets": int(len(e))}
+    summary = ensemble_summary(r, not args.no_graph)
     out = {
         "metric": "ensemble particle-steps/s (batched vs serial)",
-        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_batched / args.steps,
+        "value": r["value"], "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r["ms_batched"] / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic parabolic two-disk encounters (seeded generator)",
-        "config": {"workload": f"ensemble of {args.systems} two-disk encounters, {n} particles "
-                               f"(n_s log-uniform 2048..65536), Barnes-Hut rebuilt every step, "
-                               f"(system, 30-bit) keys, theta={theta}, eps={eps}, dt={dt}, "
-                               f"wave64 persistent walk, "
-                               f"{'eager' if args.no_graph else 'graph-captured'} step",
-                   "systems": args.systems, "n": n, "theta": theta},
-        "serial": {"value": serial_value, "ms_per_ensemble_step": ms_serial,
-                   "how": "each system alone on the plain path (bh, wave64, 30-bit keys, "
-                          + ("eager" if args.no_graph else "graph mode") + "), summed"},
-        "batched_over_serial": value / serial_value,
-        "autotune": tuned,
-        "batched_forces_bitwise_equal_serial": not mismatched,
-        "phase_ms": {k: phase[i] / max(calls, 1) for i, k in enumerate(names)},
-        "build_roofline_hbm": build_roofline(phase, calls, n, 63, passes=5),
+        "config": {"workload": summary["workload"], "systems": args.systems, "n": r["n"],
+                   "theta": r["theta"]},
+        "serial": {"value": r["serial_value"], "ms_per_ensemble_step": r["ms_serial"],
+                   "how": summary["serial_how"]},
+        "batched_over_serial": summary["batched_over_serial"],
+        "autotune": r["tuned"],
+        "batched_forces_bitwise_equal_serial": r["bitwise"],
+        "phase_ms": {k: r["phase"][i] / max(r["calls"], 1) for i, k in enumerate(names)},
+        "build_roofline_hbm": build_roofline(r["phase"], r["calls"], r["n"], 63, passes=5),
         "roofline": {"bound": "valu", "kernel": "walk_items",
                      "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
-                     "flops_per_launch": flops, "walk_ms": walk_ms,
+                     "flops_per_launch": r["flops"], "walk_ms": r["walk_ms"],
                      "note": f"walk duration from {args.phase_steps} eager steps after the "
                              f"timed graph steps"},
-        "accuracy": accuracy,
+        "accuracy": r["accuracy"],
         "cpu_baseline": None,
     }
     print(json.dumps(out))
+
+
+def allpairs_probe(pos, eps, launches=1):
+    """Tiled all-pairs on the same positions: one warm-up launch, then
+    `launches` timed force evaluations (HIP events around the kernels);
+    20 flop per interaction, n^2 interactions per launch (main.cu:203-207)."""
+    from nbodyssey_amd import engine
+    n = len(pos)
+    with engine.Simulation(pos, eps=eps, force="naive") as sim:
+        sim.compute_forces()
+        sim.reset_force_timing()
+        for _ in range(launches):
+            sim.compute_forces()
+        ms = sim.force_ms_total() / max(sim.force_calls(), 1)
+    gint = n * n / (ms * 1e-3) / 1e9
+    return {"n": n, "launches": launches, "ms_per_launch": ms, "ginteractions_per_s": gint,
+            "achieved_tflops": gint * 20.0 / 1e3, "peak_tflops": FP32_PEAK_TFLOPS,
+            "frac_of_fp32_peak": gint * 20.0 / 1e3 / FP32_PEAK_TFLOPS,
+            "convention": "20 flop per interaction, n^2 interactions (main.cu:203-207)"}
 
 
 def precision_main(args):
@@ -318,8 +406,18 @@ def main():
     ap.add_argument("--traverse", default="warp", choices=["warp", "thread"])
     ap.add_argument("--group-width", type=int, default=64)
     ap.add_argument("--key-bits", type=int, default=30)
-    ap.add_argument("--cpu-sample", type=int, default=16384)
+    ap.add_argument("--cpu-sample", type=int, default=65536,
+                    help="particles of the one timed CPU step extrapolated to n")
+    ap.add_argument("--cpu-c1-budget", type=float, default=15.0,
+                    help="seconds of the 12,000-particle CPU run (up to its 200 steps)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="two-disk default: skip the all-pairs and ensemble measurements "
+                         "that the default line carries beside the headline")
+    ap.add_argument("--ens-steps", type=int, default=50,
+                    help="default line: batched ensemble steps timed")
+    ap.add_argument("--ens-serial-steps", type=int, default=5,
+                    help="default line: timed steps per system of the serial ensemble run")
     ap.add_argument("--accuracy-targets", type=int, default=256)
     ap.add_argument("--far16", action="store_true",
                     help="FP16 far field for accepted cells (wave64 walk precision mode 2); "
@@ -375,8 +473,16 @@ def main():
                        f"30%) on a parabolic orbit, {n} particles",
                        f"particle-steps/s @{n} composite BH theta={args.theta} incl. rebuild",
                        "synthetic composite-galaxy encounter (seeded generator)",
-                       energy_theta=args.energy_theta, cpu=False)
+                       energy_theta=args.energy_theta)
     eps, dt = 0.02, 0.005
+    headline = args.force == "bh" and not args.headline_only
+    # the CPU baseline forks its worker pool, so it runs before the GPU is touched
+    cpu_line = None
+    if args.force == "bh" and not args.no_cpu_baseline:
+        t_cpu = time.perf_counter()
+        cpu_line = cpu_baseline(args.n, args.cpu_sample, args.cpu_c1_budget)
+        print(f"bench: cpu baseline {time.perf_counter() - t_cpu:.1f}s", file=sys.stderr,
+              flush=True)
     t_ic = time.perf_counter()
     pos, vel = ic.two_disk_encounter(args.n, seed=1, eps=eps)
     print(f"bench: n={args.n} ic {time.perf_counter() - t_ic:.1f}s", file=sys.stderr, flush=True)
@@ -385,16 +491,20 @@ def main():
         sim = engine.Simulation(pos, vel, dt=dt, eps=eps, force="naive")
         return naive_main(args, sim, eps)
     return bh_main(args, pos, vel, eps, dt, f"two-disk {args.n} particles",
-                   "particle-steps/s @1M BH θ=0.5 incl. rebuild",
+                   "particle-steps/s @1M BH θ=0.5 incl. rebuild; ensemble vs serial at ≤1% "
+                   "force err",
                    "synthetic two-disk encounter (seeded generator)",
-                   energy_theta=None, cpu=not args.no_cpu_baseline)
+                   energy_theta=None, cpu_line=cpu_line, headline=headline)
 
 
-def bh_main(args, pos, vel, eps, dt, what, metric, data, energy_theta=None, cpu=True):
+def bh_main(args, pos, vel, eps, dt, what, metric, data, energy_theta=None, cpu_line=None,
+            headline=False):
     """Barnes-Hut leapfrog steps of one system: timed steps, phase times,
     walk roofline from the per-target walk counts, accuracy vs FP64 direct
     summation, optional tree-energy drift over the timed steps and the CPU
-    baseline."""
+    baseline.  headline: also the FP64 energy drift over the timed steps, one
+    all-pairs launch on the initial state and the batched ensemble against
+    its serial runs (the three targets the metric names)."""
     from nbodyssey_amd import engine
     n = len(pos)
     sim = engine.Simulation(pos, vel, dt=dt, eps=eps, theta=args.theta, force="bh",
@@ -409,6 +519,8 @@ def bh_main(args, pos, vel, eps, dt, what, metric, data, energy_theta=None, cpu=
     energy = None
     if energy_theta:
         e0 = sim.total_energy_tree(energy_theta)
+    elif headline:
+        e0 = sim.total_energy()   # exact FP64 sums, untimed
 
     sim.reset_force_timing()
     sim.synchronize()
@@ -421,10 +533,11 @@ def bh_main(args, pos, vel, eps, dt, what, metric, data, energy_theta=None, cpu=
     value = n * args.steps / (ms_events * 1e-3)
     print(f"bench: {args.steps} steps {ms_events:.2f} ms (wall {wall * 1e3:.2f} ms)",
           file=sys.stderr, flush=True)
-    if energy_theta:
-        e1 = sim.total_energy_tree(energy_theta)
-        energy = {"method": f"tree potential at theta={energy_theta} (nb_total_energy_tree)",
-                  "E_start": float(e0[0]), "E_end": float(e1[0]),
+    if energy_theta or headline:
+        e1 = sim.total_energy_tree(energy_theta) if energy_theta else sim.total_energy()
+        how = (f"tree potential at theta={energy_theta} (nb_total_energy_tree)" if energy_theta
+               else "exact O(n^2) FP32 pair potentials with FP64 sums (nb_total_energy)")
+        energy = {"method": how, "E_start": float(e0[0]), "E_end": float(e1[0]),
                   "rel_drift": float((e1[0] - e0[0]) / abs(e0[0])), "steps": args.steps}
 
     # algorithmic flops of the walk on the final tree: per-target walk counts
@@ -444,7 +557,13 @@ def bh_main(args, pos, vel, eps, dt, what, metric, data, energy_theta=None, cpu=
                 "p99_rel_force_err_vs_fp64": float(np.quantile(err, 0.99)),
                 "targets": int(args.accuracy_targets),
                 "reference": "FP64 direct summation on the GPU (nb_direct_f64)"}
-    cpu_line = cpu_baseline(n, args.cpu_sample, eps, dt) if cpu else None
+    extras = {}
+    if headline:
+        sim.close()
+        print("bench: all-pairs launch", file=sys.stderr, flush=True)
+        extras["allpairs"] = allpairs_probe(pos, eps)
+        extras["ensemble"] = ensemble_summary(
+            ensemble_run(args, args.ens_steps, args.ens_serial_steps, 1), not args.no_graph)
 
     names = ["bbox", "morton", "sort", "build", "com", "traverse"]
     build = build_roofline(phase, calls, n, args.key_bits)
@@ -491,6 +610,7 @@ def bh_main(args, pos, vel, eps, dt, what, metric, data, energy_theta=None, cpu=
     }
     if energy is not None:
         out["energy"] = energy
+    out.update(extras)
     print(json.dumps(out))
     sim.close()
 

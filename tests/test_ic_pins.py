@@ -225,3 +225,16 @@ def test_numpy_kdk_run_matches_oracle_steps():
     assert np.abs(p1 - p2[:, :3]).max() < 1e-5
     dv = np.linalg.norm(v1 - v2[:, :3], axis=1) / np.linalg.norm(v2[:, :3], axis=1)
     assert np.median(dv) < 1e-5
+
+
+def test_numpy_pool_matches_serial():
+    """The pooled CPU baseline (one target block per worker) computes what
+    the serial restatement computes."""
+    from multiprocessing import get_context
+    pos, _ = ic.two_disk_encounter(3000, seed=13, eps=0.05)
+    p, m = pos[:, :3].copy(), pos[:, 3].copy()
+    eps2, g = np.float32(0.0025), np.float32(1.0)
+    a1 = numpy_ref.accelerations(p, m, eps2, g, chunk=512)
+    with get_context("fork").Pool(3) as pool:
+        a2 = numpy_ref.accelerations(p, m, eps2, g, chunk=512, pool=pool)
+    np.testing.assert_allclose(a2, a1, rtol=1e-6, atol=0)
