@@ -138,7 +138,9 @@ int nb_set_graph(nb_engine* h, int on);
 int nb_bh_release(nb_engine* h);
 
 /* Runs nsteps leapfrog steps bracketed by two HIP events on the engine stream
- * and returns the elapsed milliseconds (no host synchronisation inside). */
+ * and returns the elapsed milliseconds (no host synchronisation inside).  No
+ * per-call or per-phase events are recorded inside the bracket, so these
+ * steps are not added to nb_force_ms_total / nb_force_calls / nb_bh_phase_ms. */
 int nb_time_steps(nb_engine* h, int nsteps, double* ms);
 
 /* ---- stage-level entry points (the reference keeps these stages in an
