@@ -130,7 +130,7 @@ BhScratch carve_bh_scratch(void* base, int n, size_t* bytes) {
   s.bbox_partial = reinterpret_cast<float4*>(take(2 * (size_t)kBboxBlocks * sizeof(float4)));
   s.max_tiles = sort_max_tiles((int)c);
   s.lookback = reinterpret_cast<unsigned long long*>(
-      take((size_t)kMaxPasses * s.max_tiles * 256 * sizeof(unsigned long long)));
+      take((size_t)kHistWords * s.max_tiles * sizeof(unsigned long long)));
   s.keys_a = reinterpret_cast<uint64_t*>(take(c * sizeof(uint64_t)));
   s.keys_b = reinterpret_cast<uint64_t*>(take(c * sizeof(uint64_t)));
   s.vals_a = reinterpret_cast<unsigned*>(take(c * sizeof(unsigned)));
@@ -158,13 +158,12 @@ void enqueue_bh_force(hipStream_t stream, const BhScratch& b, float4* pos, float
     launch_bbox(stream, pos, n, b.bbox_partial);
   mark(0);
   launch_morton_hist(stream, pos, n, b.bbox_partial, bbox_parts(n), b.quant, p.key_bits,
-                     b.keys_a, b.state);
+                     sort_input(p.key_bits, b.keys_a, b.keys_b), b.state);
   mark(1);
   launch_sort(stream, p.key_bits, b.keys_a, b.keys_b, b.vals_a, b.vals_b, n, b.state, b.lookback,
               b.max_tiles, true, pos, b.pos_sorted);
   mark(2);
-  launch_karras(stream, p.key_bits, b.keys_a, n, b.nodes, b.up, b.visit, b.state,
-                sort_passes(p.key_bits));
+  launch_karras(stream, p.key_bits, b.keys_a, n, b.nodes, b.up, b.visit, b.state);
   mark(3);
   launch_com(stream, b.pos_sorted, n, b.nodes, b.up, b.visit, b.box_lo, b.box_hi, b.frontier,
              &b.state->error);
@@ -332,8 +331,8 @@ class Engine {
     return w;
   }
 
-  // Sorted keys / original indices after the last sort pass (an ensemble
-  // sorts an odd number of passes and ends in the b buffers).
+  // Sorted keys / original indices after the last sort pass (the b buffers
+  // for an odd number of passes).
   uint64_t* keys_final() { return (nsys > 0 && (ens_passes() & 1)) ? keys_b.p : keys_a.p; }
   unsigned* perm_final() { return (nsys > 0 && (ens_passes() & 1)) ? vals_b.p : vals_a.p; }
 
@@ -787,10 +786,10 @@ class Engine {
     quant.alloc(1, true);
     state.alloc(1, true);
     max_tiles = sort_max_tiles(n);
-    lookback.alloc((size_t)kMaxPasses * max_tiles * 256, true);
+    lookback.alloc((size_t)kHistWords * max_tiles, true);
     // the sort state is reset by each step's Karras kernel for the next step;
     // start from a reset state on this stream
-    launch_sort_init(stream, state.p, kMaxPasses);
+    launch_sort_init(stream, state.p);
     if (nsys > 0) launch_ens_box_init(stream, sys_box.p, nsys);
     bh_cap = n;
   }
@@ -1407,8 +1406,6 @@ int nb_morton(const float* pos4, int n, int key_bits, uint64_t* keys) {
     part.alloc(2 * nb::kBboxBlocks);
     qp.alloc(1, true);
     ss.alloc(1, true);
-    const int passes = nb::sort_passes(key_bits);
-    (void)passes;
     nb::launch_bbox(nullptr, dpos.p, n, part.p);
     nb::launch_morton_hist(nullptr, dpos.p, n, part.p, nb::bbox_parts(n), qp.p, key_bits,
                            dkeys.p, ss.p);
@@ -1430,7 +1427,6 @@ int nb_sort_pairs(const uint64_t* keys, int n, int key_bits, uint64_t* keys_sort
     if (n < 1) throw nb::Error(nb::kErrInvalid, "n must be >= 1");
     if (key_bits != 30 && key_bits != 63) throw nb::Error(nb::kErrInvalid, "key_bits must be 30 or 63");
     nb::check_device();
-    const int passes = nb::sort_passes(key_bits);
     const int tiles = nb::sort_max_tiles(n);
     nb::DevBuf<uint64_t> ka, kb;
     nb::DevBuf<unsigned> va, vb;
@@ -1441,19 +1437,20 @@ int nb_sort_pairs(const uint64_t* keys, int n, int key_bits, uint64_t* keys_sort
     va.alloc(n);
     vb.alloc(n);
     ss.alloc(1, true);
-    lb.alloc((size_t)nb::kMaxPasses * tiles * 256, true);
+    lb.alloc((size_t)nb::kHistWords * tiles, true);
+    void* in = nb::sort_input(key_bits, ka.p, kb.p);
     if (key_bits == 30) {
       std::vector<uint32_t> k(n);
       for (int i = 0; i < n; ++i) {
         if (keys[i] >> 30) throw nb::Error(nb::kErrInvalid, "30-bit sort given a key >= 2^30");
         k[i] = (uint32_t)keys[i];
       }
-      NB_HIP(hipMemcpy(ka.p, k.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+      NB_HIP(hipMemcpy(in, k.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
     } else {
-      NB_HIP(hipMemcpy(ka.p, keys, n * sizeof(uint64_t), hipMemcpyHostToDevice));
+      NB_HIP(hipMemcpy(in, keys, n * sizeof(uint64_t), hipMemcpyHostToDevice));
     }
-    nb::launch_sort_init(nullptr, ss.p, passes);
-    nb::launch_hist(nullptr, ka.p, n, key_bits, ss.p);
+    nb::launch_sort_init(nullptr, ss.p);
+    nb::launch_hist(nullptr, in, n, key_bits, ss.p);
     nb::launch_sort(nullptr, key_bits, ka.p, kb.p, va.p, vb.p, n, ss.p, lb.p, tiles, true,
                     nullptr, nullptr);
     NB_HIP(hipDeviceSynchronize());
@@ -1492,7 +1489,7 @@ int nb_karras_build(const uint64_t* keys_sorted, int n, int key_bits, int32_t* c
     nodes.alloc(n - 1);
     par.alloc(2 * (size_t)n - 1);
     visit.alloc(n - 1);
-    nb::launch_karras(nullptr, key_bits, dk.p, n, nodes.p, par.p, visit.p, nullptr, 0);
+    nb::launch_karras(nullptr, key_bits, dk.p, n, nodes.p, par.p, visit.p, nullptr);
     NB_HIP(hipDeviceSynchronize());
     if (parent) {
       std::vector<int4> u(2 * (size_t)n - 1);

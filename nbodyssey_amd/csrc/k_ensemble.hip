@@ -103,15 +103,19 @@ __global__ __launch_bounds__(kBlock) void ens_kick_drift_bbox_kernel(
 }
 
 constexpr int kEnsMaxSys = 1024;
-constexpr int kEnsPasses = 5;  // keys < 2^40
+// (system, key) words are below 2^40: five passes of 8-bit digits
+constexpr int kEnsBits = kSortBits;
+constexpr int kEnsPasses = 5;
+constexpr int kEnsWords = kEnsPasses << kEnsBits;
+static_assert(kEnsWords <= kHistWords && kEnsPasses * kEnsBits >= 40, "ensemble digit plan");
 
 __global__ __launch_bounds__(kBlock) void ens_morton_hist_kernel(
     const float4* __restrict__ pos, int n, const int* __restrict__ off, int nsys,
     const unsigned* __restrict__ box, uint64_t* __restrict__ keys, SortState* ss) {
   __shared__ int s_off[kEnsMaxSys + 1];
   __shared__ double s_q[kEnsMaxSys][4];
-  __shared__ unsigned h[kEnsPasses][256];
-  for (int p = 0; p < kEnsPasses; ++p) h[p][threadIdx.x] = 0;
+  __shared__ unsigned h[kEnsWords];
+  for (int i = threadIdx.x; i < kEnsWords; i += blockDim.x) h[i] = 0;
   for (int s = threadIdx.x; s <= nsys; s += blockDim.x) s_off[s] = off[s];
   for (int s = threadIdx.x; s < nsys; s += blockDim.x) {
     // FP64 origin and cubic scale of system s (lbvh_check.py:45-48)
@@ -136,12 +140,13 @@ __global__ __launch_bounds__(kBlock) void ens_morton_hist_kernel(
     keys[i] = k;
 #pragma unroll
     for (int q = 0; q < kEnsPasses; ++q)
-      atomicAdd(&h[q][(unsigned)(k >> (8 * q)) & 255u], 1u);
+      atomicAdd(&h[(q << kEnsBits) + ((unsigned)(k >> (kEnsBits * q)) & ((1u << kEnsBits) - 1u))],
+                1u);
   }
   __syncthreads();
-  for (int q = 0; q < kEnsPasses; ++q) {
-    const unsigned c = h[q][threadIdx.x];
-    if (c) atomic_add_relaxed(&ss->hist[q][threadIdx.x], c);
+  for (int i = threadIdx.x; i < kEnsWords; i += blockDim.x) {
+    const unsigned c = h[i];
+    if (c) atomic_add_relaxed(&ss->hist[i], c);
   }
 }
 
@@ -163,7 +168,7 @@ __global__ __launch_bounds__(kBlock) void ens_karras_kernel(
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (blockIdx.x == 0) {
     // this step's sort is done and its boxes consumed: reset for the next
-    for (int p = 0; p < kEnsPasses; ++p) ss->hist[p][threadIdx.x] = 0;
+    for (int k = threadIdx.x; k < kHistWords; k += blockDim.x) ss->hist[k] = 0;
     if (threadIdx.x < kMaxPasses) ss->ticket[threadIdx.x] = 0;
     if (threadIdx.x == 0) {
       ss->epoch = ss->epoch + 1;

@@ -10,11 +10,17 @@
 //
 // Sort (replaces CUB SortPairs, forces_bh.cu:490-497): LSD radix sort, 8-bit
 // digits, one kernel per pass.  Each 256-thread workgroup takes a ticket-
-// ordered tile of 4096 keys, ranks its keys stably in LDS (per-wave ballot
-// match of equal digits), publishes per-digit tile counts and resolves its
-// global offsets by decoupled look-back over earlier tiles only (earlier
-// tickets are already resident, so the wait always ends), then scatters.
-// Stable: ties keep input order, so equal keys end in ascending original index.
+// ordered tile of keys, ranks its keys stably in LDS (per-wave ballot match
+// of equal digits), publishes per-digit tile counts and resolves its global
+// offsets by decoupled look-back over earlier tiles only (earlier tickets
+// are already resident, so the wait always ends).  The tile is then staged
+// in LDS in sorted order and written out in that order, so consecutive
+// threads write consecutive positions of each digit's run (1M 30-bit keys:
+// 95.5 us for the four passes against 103.8 scattering from registers).
+// Stable: ties keep input order, so equal keys end in ascending original
+// index.  A 1M-key pass is one tile per CU, bound by latency: ~5 us to load
+// and rank, ~9 us until inclusive prefixes have propagated through the 256
+// tiles, ~2 us to write, ~3 us between launches.
 //
 // Karras build (forces_bh.cu:234-277): one thread per internal node.
 // COM (forces_bh.cu:281-331): leaves climb towards the root and the second
@@ -39,16 +45,42 @@ __device__ __forceinline__ K make_key(float4 p, double lx, double ly, double lz,
   return (K)k;
 }
 
+// Digit plan of a key word: 32-bit words hold 30-bit keys, 64-bit words 63.
+template <typename K>
+struct Digits {
+  static constexpr int kKeyBits = sizeof(K) == 4 ? 30 : 63;
+  static constexpr int kBits = sort_bits(kKeyBits);
+  static constexpr int kPasses = sort_passes(kKeyBits);
+  static constexpr int kWords = kPasses << kBits;
+  static_assert(kWords <= kHistWords && kPasses * kBits >= kKeyBits, "digit plan");
+};
+
+// LDS histogram of every pass (pass p at p << bits) for keys k.
+template <typename K>
+__device__ __forceinline__ void count_digits(unsigned* h, K k) {
+  constexpr int B = Digits<K>::kBits;
+#pragma unroll
+  for (int p = 0; p < Digits<K>::kPasses; ++p)
+    atomicAdd(&h[(p << B) + ((unsigned)(k >> (B * p)) & ((1u << B) - 1u))], 1u);
+}
+
+template <typename K>
+__device__ __forceinline__ void flush_digits(const unsigned* h, SortState* ss) {
+  for (int i = threadIdx.x; i < Digits<K>::kWords; i += kBlock) {
+    const unsigned c = h[i];
+    if (c) atomic_add_relaxed(&ss->hist[i], c);
+  }
+}
+
 template <typename K>
 __global__ __launch_bounds__(kBlock) void morton_hist_kernel(const float4* __restrict__ pos, int n,
                                                              const float4* __restrict__ partial,
                                                              int nparts, QuantParams* qp,
-                                                             K* __restrict__ keys, SortState* ss,
-                                                             int passes) {
-  __shared__ unsigned h[kMaxPasses][256];
+                                                             K* __restrict__ keys, SortState* ss) {
+  __shared__ unsigned h[Digits<K>::kWords];
   __shared__ float s_box[kBlock / 64][6];
   __shared__ double s_q[4];
-  for (int p = 0; p < passes; ++p) h[p][threadIdx.x] = 0;
+  for (int i = threadIdx.x; i < Digits<K>::kWords; i += kBlock) h[i] = 0;
   // global bounding box from the per-workgroup partials (forces_bh.cu:158-188)
   float l[3] = {3.402823466e38f, 3.402823466e38f, 3.402823466e38f};
   float u[3] = {-3.402823466e38f, -3.402823466e38f, -3.402823466e38f};
@@ -98,58 +130,72 @@ __global__ __launch_bounds__(kBlock) void morton_hist_kernel(const float4* __res
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     K k = make_key<K>(pos[i], lx, ly, lz, inv);
     keys[i] = k;
-    for (int p = 0; p < passes; ++p) atomicAdd(&h[p][(unsigned)(k >> (8 * p)) & 255u], 1u);
+    count_digits<K>(h, k);
   }
   lds_barrier();  // the histogram; the key stores are for later launches
-  for (int p = 0; p < passes; ++p) {
-    unsigned c = h[p][threadIdx.x];
-    if (c) atomic_add_relaxed(&ss->hist[p][threadIdx.x], c);
-  }
+  flush_digits<K>(h, ss);
 }
 
 template <typename K>
 __global__ __launch_bounds__(kBlock) void hist_kernel(const K* __restrict__ keys, int n,
-                                                      SortState* ss, int passes) {
-  __shared__ unsigned h[kMaxPasses][256];
-  for (int p = 0; p < passes; ++p) h[p][threadIdx.x] = 0;
+                                                      SortState* ss) {
+  __shared__ unsigned h[Digits<K>::kWords];
+  for (int i = threadIdx.x; i < Digits<K>::kWords; i += kBlock) h[i] = 0;
   __syncthreads();
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    K k = keys[i];
-    for (int p = 0; p < passes; ++p) atomicAdd(&h[p][(unsigned)(k >> (8 * p)) & 255u], 1u);
-  }
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    count_digits<K>(h, keys[i]);
   __syncthreads();
-  for (int p = 0; p < passes; ++p) {
-    unsigned c = h[p][threadIdx.x];
-    if (c) atomic_add_relaxed(&ss->hist[p][threadIdx.x], c);
-  }
+  flush_digits<K>(h, ss);
 }
 
-__global__ __launch_bounds__(kBlock) void sort_init_kernel(SortState* ss, int passes) {
-  for (int p = 0; p < passes; ++p) ss->hist[p][threadIdx.x] = 0;
+// Histograms and tickets to zero, a new look-back epoch (one workgroup).
+__device__ __forceinline__ void reset_sort_state(SortState* ss) {
+  for (int i = threadIdx.x; i < kHistWords; i += kBlock) ss->hist[i] = 0;
   if (threadIdx.x < kMaxPasses) ss->ticket[threadIdx.x] = 0;
   if (threadIdx.x == 0) ss->epoch = ss->epoch + 1;
 }
+
+__global__ __launch_bounds__(kBlock) void sort_init_kernel(SortState* ss) { reset_sort_state(ss); }
 
 constexpr unsigned long long kFlagAgg = 1ull << 32;
 constexpr unsigned long long kFlagInc = 2ull << 32;
 constexpr unsigned kLookbackSpins = 1u << 20;
 
-template <typename K, bool FIRST, bool GATHER, int ITEMS>
+// D consecutive 32-bit words, moved as one LDS / global access.
+template <int D>
+struct alignas(4 * D) Words {
+  unsigned v[D];
+};
+
+template <typename K, int BITS, bool FIRST, bool GATHER, int ITEMS>
 __global__ __launch_bounds__(kBlock) void onesweep_kernel(
     const K* __restrict__ kin, const unsigned* __restrict__ vin, K* __restrict__ kout,
     unsigned* __restrict__ vout, int n, int pass, SortState* ss,
     unsigned long long* lookback, int max_tiles, const float4* __restrict__ gsrc,
     float4* __restrict__ gdst) {
   constexpr int WAVES = kBlock / 64;
-  __shared__ unsigned s_cnt[WAVES][256];
-  __shared__ unsigned s_goff[256];
-  __shared__ unsigned s_scan[WAVES];
+  constexpr int RADIX = 1 << BITS;
+  constexpr unsigned MASK = RADIX - 1;
+  constexpr int TILE = kBlock * ITEMS;
+  constexpr int D = RADIX / kBlock;  // consecutive digits owned by each thread
+  static_assert(D * kBlock == RADIX && (D == 1 || D == 2 || D == 4), "digits per thread");
+  // look-back window: earlier tiles read at once per digit (1M 30-bit keys:
+  // 95.5 us for the sort reading 4, 99.0 reading 1, 97.9 reading 8)
+  constexpr int LB = D == 1 ? 4 : 1;
+  using W = Words<D>;
+  __shared__ __align__(16) unsigned s_cnt[WAVES][RADIX];
+  __shared__ __align__(16) unsigned s_off[RADIX];
+  __shared__ K s_key[TILE];
+  __shared__ unsigned s_val[TILE];
+  __shared__ unsigned s_scan[2][WAVES];
   __shared__ unsigned s_tile;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int shift = pass * 8;
+  const int shift = pass * BITS;
   if (tid == 0) s_tile = atomic_add_relaxed(&ss->ticket[pass], 1u);
 #pragma unroll
-  for (int k = 0; k < WAVES; ++k) s_cnt[k][tid] = 0;
+  for (int k = 0; k < WAVES; ++k)
+#pragma unroll
+    for (int e = 0; e < D; ++e) s_cnt[k][e * kBlock + tid] = 0;
   __syncthreads();
   const unsigned tile = s_tile;
   if (tile >= gridDim.x) {  // tickets not reset: refuse rather than write out of range
@@ -157,7 +203,8 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
     return;
   }
   const unsigned epoch = ss->epoch & 0x3fffffffu;
-  const int base = (int)tile * (kBlock * ITEMS) + w * (64 * ITEMS);
+  const int tile0 = (int)tile * TILE;
+  const int base = tile0 + w * (64 * ITEMS);
 
   K key[ITEMS];
   unsigned val[ITEMS], rank[ITEMS];
@@ -168,13 +215,15 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
     key[j] = ok ? kin[idx] : (K)~(K)0;
     val[j] = FIRST ? (unsigned)idx : (ok ? vin[idx] : 0u);
   }
+  // the bucket sizes of this pass, for the output offsets below
+  const W hv = reinterpret_cast<const W*>(&ss->hist[pass << BITS])[tid];
   const unsigned long long lt = (1ull << lane) - 1ull;
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j) {
-    unsigned d = (unsigned)(key[j] >> shift) & 255u;
+    unsigned d = (unsigned)(key[j] >> shift) & MASK;
     unsigned long long peers = ~0ull;
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < BITS; ++b) {
       unsigned bit = (d >> b) & 1u;
       unsigned long long bal = __ballot(bit);
       peers &= bit ? bal : ~bal;
@@ -186,69 +235,163 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
   }
   __syncthreads();
 
-  // digit `tid`: per-wave exclusive offsets and the tile total
-  unsigned total = 0;
+  // digits tid*D .. tid*D + D-1: per-wave exclusive offsets and tile totals
+  W total{};
 #pragma unroll
   for (int k = 0; k < WAVES; ++k) {
-    unsigned c = s_cnt[k][tid];
-    s_cnt[k][tid] = total;
-    total += c;
+    W* row = reinterpret_cast<W*>(&s_cnt[k][0]) + tid;
+    W c = *row, o;
+#pragma unroll
+    for (int e = 0; e < D; ++e) {
+      o.v[e] = total.v[e];
+      total.v[e] += c.v[e];
+    }
+    *row = o;
   }
-  // start of bucket `tid` in the output: exclusive scan of the histogram
-  unsigned hv = ss->hist[pass][tid];
-  unsigned inc = wave_incl_scan(hv);
-  if (lane == 63) s_scan[w] = inc;
+  // exclusive scans over the digits: where each digit starts in the tile
+  // (tile totals) and in the output (the histogram)
+  unsigned tsum = 0, hsum = 0;
+#pragma unroll
+  for (int e = 0; e < D; ++e) {
+    tsum += total.v[e];
+    hsum += hv.v[e];
+  }
+  const unsigned tinc = wave_incl_scan(tsum), hinc = wave_incl_scan(hsum);
+  if (lane == 63) {
+    s_scan[0][w] = tinc;
+    s_scan[1][w] = hinc;
+  }
   __syncthreads();
-  unsigned bucket = inc - hv;
-  for (int k = 0; k < w; ++k) bucket += s_scan[k];
+  unsigned tstart[D], bucket[D];
+  tstart[0] = tinc - tsum;
+  bucket[0] = hinc - hsum;
+  for (int k = 0; k < w; ++k) {
+    tstart[0] += s_scan[0][k];
+    bucket[0] += s_scan[1][k];
+  }
+#pragma unroll
+  for (int e = 1; e < D; ++e) {
+    tstart[e] = tstart[e - 1] + total.v[e - 1];
+    bucket[e] = bucket[e - 1] + hv.v[e - 1];
+  }
+  {
+    W t;
+#pragma unroll
+    for (int e = 0; e < D; ++e) t.v[e] = tstart[e];
+    reinterpret_cast<W*>(s_off)[tid] = t;
+  }
+  __syncthreads();
+  // stage the tile in LDS in sorted order: digit start + wave offset + rank
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    const unsigned d = (unsigned)(key[j] >> shift) & MASK;
+    const unsigned at = s_off[d] + s_cnt[w][d] + rank[j];
+    s_key[at] = key[j];
+    s_val[at] = val[j];
+  }
 
-  // decoupled look-back over earlier tiles for digit `tid`
-  unsigned long long* row = lookback + (size_t)pass * max_tiles * 256;
+  // decoupled look-back over earlier tiles for this thread's digits, LB
+  // tiles at a time: the window's words are loaded together, then summed
+  // from the nearest tile back to the first inclusive count or to the
+  // first word not yet published (read again on the next round)
+  unsigned long long* row = lookback + (size_t)pass * max_tiles * RADIX + tid * D;
   const unsigned long long tag = (unsigned long long)epoch << 34;
-  unsigned excl = 0;
+  unsigned excl[D];
+#pragma unroll
+  for (int e = 0; e < D; ++e) excl[e] = 0;
   if (tile == 0) {
-    store_agent(row + tid, tag | kFlagInc | total);
+#pragma unroll
+    for (int e = 0; e < D; ++e) store_agent(row + e, tag | kFlagInc | total.v[e]);
   } else {
-    store_agent(row + (size_t)tile * 256 + tid, tag | kFlagAgg | total);
-    int k = (int)tile - 1;
+    unsigned long long* mine = row + (size_t)tile * RADIX;
+#pragma unroll
+    for (int e = 0; e < D; ++e) store_agent(mine + e, tag | kFlagAgg | total.v[e]);
+    int next[D];  // nearest earlier tile not yet summed, per digit
+#pragma unroll
+    for (int e = 0; e < D; ++e) next[e] = (int)tile - 1;
+    unsigned pending = (1u << D) - 1u;
     // Every earlier tile publishes within microseconds of this one; a word
-    // that stays stale for kLookbackSpins polls (tens of milliseconds) means
-    // the sort state was not reset, so the tile reports an error and still
+    // that stays stale for kLookbackSpins rounds (about a second) means the
+    // sort state was not reset, so the tile reports an error and still
     // publishes an inclusive count, and no later tile waits on it forever.
     unsigned spins = 0;
-    while (k >= 0) {
-      unsigned long long v = load_agent(row + (size_t)k * 256 + tid);
-      unsigned long long flag = v & (3ull << 32);
-      if ((unsigned)(v >> 34) != epoch || flag == 0) {
+    while (pending) {
+      unsigned long long v[D][LB];
+#pragma unroll
+      for (int e = 0; e < D; ++e)
+#pragma unroll
+        for (int i = 0; i < LB; ++i) {
+          const int t = next[e] - i;
+          // words before tile 0 read as unpublished; tile 0's inclusive
+          // count always ends the sum before them
+          v[e][i] = ((pending >> e) & 1u) && t >= 0 ? load_agent(row + (size_t)t * RADIX + e)
+                                                    : 0ull;
+        }
+      bool stale = false;
+#pragma unroll
+      for (int e = 0; e < D; ++e) {
+        if (!((pending >> e) & 1u)) continue;
+        bool stop = false;
+        int used = 0;
+#pragma unroll
+        for (int i = 0; i < LB; ++i) {
+          const unsigned long long x = v[e][i];
+          const unsigned long long flag = x & (3ull << 32);
+          const bool ready = (unsigned)(x >> 34) == epoch && flag != 0;
+          if (stop) continue;
+          if (!ready) {
+            stale = true;
+            stop = true;
+            continue;
+          }
+          excl[e] += (unsigned)x;
+          ++used;
+          if (flag == kFlagInc) {
+            pending &= ~(1u << e);
+            stop = true;
+          }
+        }
+        next[e] -= used;
+      }
+      if (stale) {
         if (++spins > kLookbackSpins) {
           ss->error = 1;
           break;
         }
         __builtin_amdgcn_s_sleep(1);
-        continue;
       }
-      excl += (unsigned)v;
-      if (flag == kFlagInc) break;
-      --k;
     }
-    store_agent(row + (size_t)tile * 256 + tid, tag | kFlagInc | (excl + total));
+#pragma unroll
+    for (int e = 0; e < D; ++e) store_agent(mine + e, tag | kFlagInc | (excl[e] + total.v[e]));
   }
-  s_goff[tid] = bucket + excl;
-  lds_barrier();  // s_goff; the published prefix needs no wait here
+  __syncthreads();  // every wave has staged its keys and read the tile starts
+  {
+    // s_off becomes output position - tile position per digit
+    W g;
+#pragma unroll
+    for (int e = 0; e < D; ++e) g.v[e] = bucket[e] + excl[e] - tstart[e];
+    reinterpret_cast<W*>(s_off)[tid] = g;
+  }
+  __syncthreads();
 
+  // scatter in tile order: consecutive threads write consecutive output
+  // positions within each digit's run.  The padding (all-ones keys past n)
+  // ranks after every real key of the tile, so the real keys fill [0, valid).
+  const int valid = min(TILE, n - tile0);
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j) {
-    int idx = base + j * 64 + lane;
-    if (idx < n) {
-      unsigned d = (unsigned)(key[j] >> shift) & 255u;
-      unsigned dest = s_goff[d] + s_cnt[w][d] + rank[j];
-      if (dest >= (unsigned)n || val[j] >= (unsigned)n) {  // only on a corrupt histogram
+    const int i = j * kBlock + tid;
+    if (i < valid) {
+      const K k = s_key[i];
+      const unsigned x = s_val[i];
+      const unsigned dest = s_off[(unsigned)(k >> shift) & MASK] + (unsigned)i;
+      if (dest >= (unsigned)n || x >= (unsigned)n) {  // only on a corrupt histogram
         ss->error = 1;
         continue;
       }
-      kout[dest] = key[j];
-      vout[dest] = val[j];
-      if (GATHER) gdst[dest] = gsrc[val[j]];
+      kout[dest] = k;
+      vout[dest] = x;
+      if (GATHER) gdst[dest] = gsrc[x];
     }
   }
 }
@@ -295,7 +438,7 @@ __global__ __launch_bounds__(kBlock) void karras_kernel(const K* __restrict__ ke
                                                         Node* __restrict__ nodes,
                                                         int4* __restrict__ up,
                                                         unsigned* __restrict__ visit,
-                                                        SortState* ss, int passes) {
+                                                        SortState* ss) {
   __shared__ K s_key[kKarrasStage];
   const int base = blockIdx.x * kBlock;
   const int wlo = base - kKarrasWin;
@@ -305,9 +448,7 @@ __global__ __launch_bounds__(kBlock) void karras_kernel(const K* __restrict__ ke
   }
   if (ss && blockIdx.x == 0) {
     // the sort of this step is complete: reset its state for the next one
-    for (int p = 0; p < passes; ++p) ss->hist[p][threadIdx.x] = 0;
-    if (threadIdx.x < kMaxPasses) ss->ticket[threadIdx.x] = 0;
-    if (threadIdx.x == 0) ss->epoch = ss->epoch + 1;
+    reset_sort_state(ss);
   }
   __syncthreads();
   const KeyWindow<K> key{keys, s_key, wlo};
@@ -876,7 +1017,7 @@ int grid_for(int n, int per_thread, int cap) {
   return (int)g;
 }
 
-template <typename K>
+template <typename K, int BITS>
 void sort_typed(hipStream_t s, K* ka, K* kb, unsigned* va, unsigned* vb, int n, SortState* ss,
                 unsigned long long* lookback, int max_tiles, bool first_identity,
                 const float4* gsrc, float4* gdst, int passes) {
@@ -884,6 +1025,7 @@ void sort_typed(hipStream_t s, K* ka, K* kb, unsigned* va, unsigned* vb, int n, 
   const int tile_keys = kBlock * items;
   int tiles = (n + tile_keys - 1) / tile_keys;
   if (tiles > max_tiles) throw Error(kErrState, "sort look-back buffer too small");
+  if ((passes << BITS) > kHistWords) throw Error(kErrInvalid, "sort digit plan too large");
   for (int p = 0; p < passes; ++p) {
     const K* kin = (p & 1) ? kb : ka;
     K* kout = (p & 1) ? ka : kb;
@@ -892,8 +1034,8 @@ void sort_typed(hipStream_t s, K* ka, K* kb, unsigned* va, unsigned* vb, int n, 
     bool first = p == 0 && first_identity;
     bool gather = p == passes - 1 && gsrc != nullptr;
 #define NB_SWEEP(F, G, I)                                                                  \
-  onesweep_kernel<K, F, G, I><<<tiles, kBlock, 0, s>>>(kin, vin, kout, vout, n, p, ss, lookback, \
-                                                       max_tiles, gsrc, gdst)
+  onesweep_kernel<K, BITS, F, G, I><<<tiles, kBlock, 0, s>>>(kin, vin, kout, vout, n, p, ss, \
+                                                             lookback, max_tiles, gsrc, gdst)
     if (items == kSortItemsBig) {
       if (first && gather) NB_SWEEP(true, true, kSortItemsBig);
       else if (first) NB_SWEEP(true, false, kSortItemsBig);
@@ -922,8 +1064,6 @@ void warm_lbvh() {
   NB_HIP(hipFuncGetAttributes(&a, reinterpret_cast<const void*>(com_local_kernel)));
 }
 
-int sort_passes(int key_bits) { return key_bits == 30 ? 4 : 8; }
-
 void launch_morton_hist(hipStream_t s, const float4* pos, int n, const float4* partial,
                         int nparts, QuantParams* qp, int key_bits, void* keys, SortState* ss) {
   if (n <= 0) return;
@@ -935,10 +1075,10 @@ void launch_morton_hist(hipStream_t s, const float4* pos, int n, const float4* p
   int g = grid_for(n, n >= (1 << 19) ? 16 : 8, 2048);
   if (key_bits == 30)
     morton_hist_kernel<uint32_t><<<g, kBlock, 0, s>>>(pos, n, partial, nparts, qp,
-                                                      (uint32_t*)keys, ss, 4);
+                                                      (uint32_t*)keys, ss);
   else
     morton_hist_kernel<uint64_t><<<g, kBlock, 0, s>>>(pos, n, partial, nparts, qp,
-                                                      (uint64_t*)keys, ss, 8);
+                                                      (uint64_t*)keys, ss);
   NB_LAUNCH_CHECK();
 }
 
@@ -946,14 +1086,14 @@ void launch_hist(hipStream_t s, const void* keys, int n, int key_bits, SortState
   if (n <= 0) return;
   int g = grid_for(n, 8, 2048);
   if (key_bits == 30)
-    hist_kernel<uint32_t><<<g, kBlock, 0, s>>>((const uint32_t*)keys, n, ss, 4);
+    hist_kernel<uint32_t><<<g, kBlock, 0, s>>>((const uint32_t*)keys, n, ss);
   else
-    hist_kernel<uint64_t><<<g, kBlock, 0, s>>>((const uint64_t*)keys, n, ss, 8);
+    hist_kernel<uint64_t><<<g, kBlock, 0, s>>>((const uint64_t*)keys, n, ss);
   NB_LAUNCH_CHECK();
 }
 
-void launch_sort_init(hipStream_t s, SortState* ss, int passes) {
-  sort_init_kernel<<<1, kBlock, 0, s>>>(ss, passes);
+void launch_sort_init(hipStream_t s, SortState* ss) {
+  sort_init_kernel<<<1, kBlock, 0, s>>>(ss);
   NB_LAUNCH_CHECK();
 }
 
@@ -962,26 +1102,40 @@ void launch_sort(hipStream_t s, int key_bits, void* keys_a, void* keys_b, unsign
                  int max_tiles, bool first_pass_identity, const float4* gather_src,
                  float4* gather_dst) {
   if (n <= 0) return;
-  // small inputs (identity values): one workgroup, all passes, result in the
-  // a buffers as after an even number of onesweep passes
+  const void* in = sort_input(key_bits, keys_a, keys_b);
+  // small inputs (identity values): one workgroup sorts on 8-bit digits in
+  // all passes (4 for 30-bit keys, 8 for 63-bit), result in the a buffers
   if (first_pass_identity && key_bits == 30 && n <= SmallSort<uint32_t>::kMax) {
     small_sort_kernel<uint32_t><<<1, kSmallThreads, 0, s>>>(
-        (const uint32_t*)keys_a, (uint32_t*)keys_a, vals_a, n, 4, gather_src, gather_dst);
+        (const uint32_t*)in, (uint32_t*)keys_a, vals_a, n, 4, gather_src, gather_dst);
     NB_LAUNCH_CHECK();
     return;
   }
   if (first_pass_identity && key_bits != 30 && n <= SmallSort<uint64_t>::kMax) {
     small_sort_kernel<uint64_t><<<1, kSmallThreads, 0, s>>>(
-        (const uint64_t*)keys_a, (uint64_t*)keys_a, vals_a, n, 8, gather_src, gather_dst);
+        (const uint64_t*)in, (uint64_t*)keys_a, vals_a, n, 8, gather_src, gather_dst);
     NB_LAUNCH_CHECK();
     return;
   }
-  if (key_bits == 30)
-    sort_typed<uint32_t>(s, (uint32_t*)keys_a, (uint32_t*)keys_b, vals_a, vals_b, n, ss, lookback,
-                         max_tiles, first_pass_identity, gather_src, gather_dst, 4);
-  else
-    sort_typed<uint64_t>(s, (uint64_t*)keys_a, (uint64_t*)keys_b, vals_a, vals_b, n, ss, lookback,
-                         max_tiles, first_pass_identity, gather_src, gather_dst, 8);
+  // an odd pass count starts in the b buffers and ends in a
+  const bool odd = sort_passes(key_bits) & 1;
+  if (odd && !first_pass_identity)
+    throw Error(kErrInvalid, "an odd-pass sort takes identity values");
+  if (key_bits == 30) {
+    uint32_t* a = (uint32_t*)keys_a;
+    uint32_t* b = (uint32_t*)keys_b;
+    sort_typed<uint32_t, sort_bits(30)>(s, odd ? b : a, odd ? a : b, odd ? vals_b : vals_a,
+                                        odd ? vals_a : vals_b, n, ss, lookback, max_tiles,
+                                        first_pass_identity, gather_src, gather_dst,
+                                        sort_passes(30));
+  } else {
+    uint64_t* a = (uint64_t*)keys_a;
+    uint64_t* b = (uint64_t*)keys_b;
+    sort_typed<uint64_t, sort_bits(63)>(s, odd ? b : a, odd ? a : b, odd ? vals_b : vals_a,
+                                        odd ? vals_a : vals_b, n, ss, lookback, max_tiles,
+                                        first_pass_identity, gather_src, gather_dst,
+                                        sort_passes(63));
+  }
 }
 
 void launch_sort_u64(hipStream_t s, uint64_t* keys_a, uint64_t* keys_b, unsigned* vals_a,
@@ -990,20 +1144,18 @@ void launch_sort_u64(hipStream_t s, uint64_t* keys_a, uint64_t* keys_b, unsigned
                      const float4* gather_src, float4* gather_dst) {
   if (n <= 0) return;
   if (passes < 1 || passes > kMaxPasses) throw Error(kErrInvalid, "sort pass count out of range");
-  sort_typed<uint64_t>(s, keys_a, keys_b, vals_a, vals_b, n, ss, lookback, max_tiles,
-                       first_pass_identity, gather_src, gather_dst, passes);
+  sort_typed<uint64_t, kSortBits>(s, keys_a, keys_b, vals_a, vals_b, n, ss, lookback, max_tiles,
+                                  first_pass_identity, gather_src, gather_dst, passes);
 }
 
 void launch_karras(hipStream_t s, int key_bits, const void* keys, int n, Node* nodes, int4* up,
-                   unsigned* visit, SortState* ss, int passes) {
+                   unsigned* visit, SortState* ss) {
   if (n < 2) return;
   int g = (n - 1 + kBlock - 1) / kBlock;
   if (key_bits == 30)
-    karras_kernel<uint32_t><<<g, kBlock, 0, s>>>((const uint32_t*)keys, n, nodes, up, visit, ss,
-                                                 passes);
+    karras_kernel<uint32_t><<<g, kBlock, 0, s>>>((const uint32_t*)keys, n, nodes, up, visit, ss);
   else
-    karras_kernel<uint64_t><<<g, kBlock, 0, s>>>((const uint64_t*)keys, n, nodes, up, visit, ss,
-                                                 passes);
+    karras_kernel<uint64_t><<<g, kBlock, 0, s>>>((const uint64_t*)keys, n, nodes, up, visit, ss);
   NB_LAUNCH_CHECK();
 }
 

@@ -68,6 +68,22 @@ inline int sort_max_tiles(int n) {
   return t;
 }
 constexpr int kMaxPasses = 8;
+// Sort digit plan: 8-bit digits, 4 passes for 30-bit Morton keys, 8 for
+// 63-bit keys and 5 for ensemble (system, 30-bit key) words below 2^40.
+// Wider digits take fewer passes but each pass's look-back is slower (1M
+// 30-bit keys: 3 x 10 bits 124 us, 4 x 8 bits 95.5 us; 4M 63-bit keys:
+// 7 x 9 bits 563 us, 8 x 8 bits 541 us).  The onesweep kernel takes the
+// digit width as a template parameter (up to 10 bits).
+constexpr int kSortBits = 8;
+// histogram words (pass p at p << bits), and look-back words per tile
+constexpr int kHistWords = kMaxPasses << kSortBits;
+constexpr int sort_bits(int key_bits) { return key_bits > 0 ? kSortBits : 0; }
+constexpr int sort_passes(int key_bits) { return key_bits == 30 ? 4 : 8; }
+// The buffer a sort reads its unsorted keys from: with an odd pass count the
+// passes start in the b buffers, so that the sorted result lands in a.
+inline void* sort_input(int key_bits, void* keys_a, void* keys_b) {
+  return (sort_passes(key_bits) & 1) ? keys_b : keys_a;
+}
 
 // One internal tree node, 32 bytes so a wave-uniform walk fetches it with a
 // single scalar load.  com = (x, y, z, mass); size2 = (longest AABB edge)^2;
@@ -89,9 +105,10 @@ struct QuantParams {
 };
 
 // Onesweep bookkeeping.  `epoch` tags look-back words so they never need
-// clearing; `ticket` hands out tile ids in dispatch order.
+// clearing; `ticket` hands out tile ids in dispatch order; hist holds each
+// pass's digit counts at pass * 2^bits.
 struct SortState {
-  unsigned hist[kMaxPasses][256];
+  unsigned hist[kHistWords];
   unsigned ticket[kMaxPasses];
   unsigned epoch;
   unsigned queue;           // work-item counter of the persistent ensemble walk
@@ -142,31 +159,34 @@ void launch_direct_f64(hipStream_t s, const float4* pos, int lo, int hi, const i
 // Reduces the bbox partials, forms the FP64 quantisation (block 0 stores it
 // in *qp), writes the keys and accumulates every pass's digit histogram.
 // The sort state must be reset (launch_sort_init, or the previous step's
-// Karras kernel via launch_karras(..., ss, passes)).
+// Karras kernel via launch_karras(..., ss)).  The keys go to `keys`, which
+// must be sort_input(key_bits, keys_a, keys_b) of the sort that follows.
 void launch_morton_hist(hipStream_t s, const float4* pos, int n, const float4* partial,
                         int nparts, QuantParams* qp, int key_bits, void* keys, SortState* ss);
 void launch_hist(hipStream_t s, const void* keys, int n, int key_bits, SortState* ss);
-void launch_sort_init(hipStream_t s, SortState* ss, int passes);
-// Sorts (keys_a, vals) in place through the b buffers; result lands in *_a for
-// an even number of passes.  If gather_src is set, the last pass also writes
-// gather_dst[dest] = gather_src[val].  first_pass_identity: vals_a is taken as
-// the index (not read).
+void launch_sort_init(hipStream_t s, SortState* ss);
+// Sorts the keys in sort_input(key_bits, keys_a, keys_b) with their original
+// indices (first_pass_identity: the values are the indices, not read from
+// vals_*) through both buffer pairs; the result lands in keys_a / vals_a.  If
+// gather_src is set, the last pass also writes gather_dst[dest] =
+// gather_src[val].  The histograms must already be in ss (launch_morton_hist
+// or launch_hist).  The sort sets SortState::error if a tile id or
+// destination is out of range.
 void launch_sort(hipStream_t s, int key_bits, void* keys_a, void* keys_b, unsigned* vals_a,
                  unsigned* vals_b, int n, SortState* ss, unsigned long long* lookback,
                  int max_tiles, bool first_pass_identity, const float4* gather_src,
                  float4* gather_dst);
-// (the sort sets SortState::error if a tile id or destination is out of range)
-int sort_passes(int key_bits);
-// 64-bit keys sorted on their low 8*passes bits; the result lands in the *_a
-// buffers for an even pass count and in the *_b buffers for an odd one.
+// 64-bit keys sorted on their low passes * kSortBits bits; the result lands
+// in the *_a buffers for an even pass count and in the *_b buffers for an
+// odd one.  The histograms (pass p at p << kSortBits) must already be in ss.
 void launch_sort_u64(hipStream_t s, uint64_t* keys_a, uint64_t* keys_b, unsigned* vals_a,
                      unsigned* vals_b, int n, int passes, SortState* ss,
                      unsigned long long* lookback, int max_tiles, bool first_pass_identity,
                      const float4* gather_src, float4* gather_dst);
-// Also resets `ss` (histograms of `passes` passes, tickets, epoch) for the
-// next sort when ss is not null.
+// Also resets `ss` (histograms, tickets, epoch) for the next sort when ss is
+// not null.
 void launch_karras(hipStream_t s, int key_bits, const void* keys, int n, Node* nodes, int4* up,
-                   unsigned* visit, SortState* ss, int passes);
+                   unsigned* visit, SortState* ss);
 // COM scratch: per-workgroup frontier segments and counts (ints)
 int com_scratch_ints(int n);
 void launch_com(hipStream_t s, const float4* pos_sorted, int n, Node* nodes, const int4* up,
@@ -224,7 +244,7 @@ void launch_ens_kick_drift_bbox(hipStream_t s, float4* pos, float4* vel, const f
                                 int nsys, unsigned* box);
 void launch_ens_morton_hist(hipStream_t s, const float4* pos, int n, const int* off, int nsys,
                             const unsigned* box, uint64_t* keys, SortState* ss);
-// Also resets the sort state (ens_passes() passes), the walk queue and the
+// Also resets the sort state, the walk queue and the
 // boxes for the next step, and writes roots[s] = the internal node whose leaf
 // range is exactly system s.
 void launch_ens_karras(hipStream_t s, const uint64_t* keys, int n, const int* off, int nsys,
@@ -253,7 +273,7 @@ struct BhScratch {
   float4* bbox_partial; // 2 * kBboxBlocks
   QuantParams* quant;
   SortState* state;     // reset by launch_sort_init before the first pass
-  unsigned long long* lookback;  // kMaxPasses * max_tiles * 256
+  unsigned long long* lookback;  // kHistWords * max_tiles
   int max_tiles;
 };
 // The buffers for n particles carved out of one block at `base` in 256-byte

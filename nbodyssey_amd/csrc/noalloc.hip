@@ -54,7 +54,7 @@ const BhScratch& device_buffers(int dev, hipStream_t s) {
     throw Error(kErrState, "static scratch too small for NB_NOALLOC_MAX_N particles");
   launch_fill_u32(s, reinterpret_cast<unsigned*>(b.state), 0u, sizeof(SortState) / 4);
   launch_fill_u32(s, reinterpret_cast<unsigned*>(b.lookback), 0u,
-                  (size_t)kMaxPasses * b.max_tiles * 256 * 2);
+                  (size_t)kHistWords * b.max_tiles * 2);
   g_bufs[dev] = b;
   g_ready[dev] = true;
   return g_bufs[dev];
@@ -96,7 +96,7 @@ extern "C" int nb_compute_forces_noalloc(const float* d_pos4, int n, const nb_pa
     if (p.force == 1 && n >= 2) {
       // histograms and tile tickets start at zero, whatever an earlier call
       // on this device left behind
-      launch_sort_init(s, b.state, kMaxPasses);
+      launch_sort_init(s, b.state);
       enqueue_bh_force(s, b, pos, nullptr, acc, n, p, false, nullptr);
     } else if (p.force == 1) {
       // n < 2: zero force (forces_bh.cu:466-469)
