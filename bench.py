@@ -410,6 +410,8 @@ def main():
                          "per interaction, n^2 interactions per launch)")
     ap.add_argument("--traverse", default="warp", choices=["warp", "thread"])
     ap.add_argument("--group-width", type=int, default=64)
+    ap.add_argument("--walk-stack", type=int, default=0,
+                    help="LDS stack entries per wave of the group walk (0: automatic)")
     ap.add_argument("--key-bits", type=int, default=30)
     ap.add_argument("--cpu-sample", type=int, default=65536,
                     help="particles of the one timed CPU step extrapolated to n")
@@ -515,6 +517,8 @@ def bh_main(args, pos, vel, eps, dt, what, metric, data, energy_theta=None, cpu_
     sim = engine.Simulation(pos, vel, dt=dt, eps=eps, theta=args.theta, force="bh",
                             traverse=args.traverse, group_width=args.group_width,
                             key_bits=args.key_bits, far16=args.far16)
+    if args.walk_stack:
+        sim.set_launch(walk_stack=args.walk_stack)
     if args.graph:
         sim.set_graph(True)
     sim.compute_forces()              # a(x0)
@@ -601,7 +605,8 @@ def bh_main(args, pos, vel, eps, dt, what, metric, data, energy_theta=None, cpu_
                                f"{args.key_bits}-bit keys, theta={args.theta}, eps={eps}, dt={dt}, "
                                f"walk={args.traverse}{'' if width == 1 else width}",
                    "n": n, "theta": args.theta, "key_bits": args.key_bits,
-                   "traverse": args.traverse, "group_width": width},
+                   "traverse": args.traverse, "group_width": width,
+                   "walk_stack": sim.launch()["walk_stack"] or 1024},
         "phase_ms": {k: phase[i] / max(calls, 1) for i, k in enumerate(names)},
         "build_roofline_hbm": build,
         "step_roofline": {"t_roof_ms": t_roof_ms, "t_meas_ms": ms_per_step,

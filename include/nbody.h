@@ -180,18 +180,42 @@ int nb_karras_build(const uint64_t* keys_sorted, int n, int key_bits, int32_t* c
  * did not. */
 int nb_total_energy_tree(nb_engine* h, float theta, double out[3]);
 
+/* Launch configuration (no counterpart in the reference): kernel shapes
+ * that change speed, not the physics.  0 selects the automatic value.
+ *   allpairs_split  source partitions per 256-target block, [0, 64];
+ *   allpairs_tile   sources per LDS tile of the all-pairs kernel: 256, 512
+ *                   or 1024 (plain FP32 sums are the same bits at every
+ *                   width; compensated sums Kahan-add per tile);
+ *   walk_stack      LDS stack entries per wave of the wave64 group walk:
+ *                   512, 768, 1024 (automatic) or 2048.  A shallower stack
+ *                   lets more waves share a CU; a walk that nears its top
+ *                   pops one node at a time (same result, slower);
+ *   walk_grid       workgroups of the persistent ensemble walk (0: all that
+ *                   fit the GPU at once).
+ * Setting it drops a captured step graph (re-captured on the next step). */
+typedef struct nb_launch {
+  int allpairs_split;
+  int allpairs_tile;
+  int walk_stack;
+  int walk_grid;
+} nb_launch;
+int nb_set_launch(nb_engine* h, const nb_launch* l);
+int nb_get_launch(const nb_engine* h, nb_launch* l);
+
 /* Autotuner (no counterpart in the reference): times candidate launch
  * configurations for `steps` leapfrog steps each on the current state, and
  * keeps the fastest whose median force error against FP64 direct summation
  * (256 fixed targets) is <= max_median_err (if none is, the most accurate).
  * Candidates: Barnes-Hut group walk -- group width 64/32/16 x key bits
- * 30/63 x graph replay off/on; Barnes-Hut per-target walk (traverse THREAD,
- * whose approximation is kept) -- key bits x graph; all-pairs -- source
- * split x graph; ensemble -- persistent-walk workgroups x graph.  The
- * particle state is saved and restored (tuning does not advance the
- * simulation), also when a candidate fails, in which case the engine keeps
- * the caller's parameters; otherwise the choice stays set on the engine.
- * log (may be NULL) receives one line per candidate, the chosen one first. */
+ * 30/63 x graph replay off/on, and at width 64 the walk's LDS stack depth
+ * 512/768/1024/2048; Barnes-Hut per-target walk (traverse THREAD, whose
+ * approximation is kept) -- key bits x graph; all-pairs -- source split x
+ * LDS tile width 256/512/1024 x graph; ensemble -- persistent-walk
+ * workgroups x graph.  The particle state is saved and restored (tuning
+ * does not advance the simulation), also when a candidate fails, in which
+ * case the engine keeps the caller's parameters and launch configuration;
+ * otherwise the choice stays set on the engine.  log (may be NULL) receives
+ * one line per candidate, the chosen one first. */
 typedef struct nb_tune_result {
   int group_width;
   int key_bits;
@@ -201,6 +225,8 @@ typedef struct nb_tune_result {
   double ms_per_step;  /* of the chosen configuration */
   double median_err;   /* of the chosen configuration */
   int candidates;
+  int allpairs_tile;   /* 0: automatic (256) */
+  int walk_stack;      /* 0: automatic (1024) */
 } nb_tune_result;
 int nb_autotune(nb_engine* h, int steps, float max_median_err, nb_tune_result* out, char* log,
                 int log_len);

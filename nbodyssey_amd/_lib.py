@@ -51,6 +51,18 @@ class NbTuneResult(ctypes.Structure):
         ("ms_per_step", ctypes.c_double),
         ("median_err", ctypes.c_double),
         ("candidates", ctypes.c_int),
+        ("allpairs_tile", ctypes.c_int),
+        ("walk_stack", ctypes.c_int),
+    ]
+
+
+class NbLaunch(ctypes.Structure):
+    """nb_launch: kernel shapes (0 = automatic)."""
+    _fields_ = [
+        ("allpairs_split", ctypes.c_int),
+        ("allpairs_tile", ctypes.c_int),
+        ("walk_stack", ctypes.c_int),
+        ("walk_grid", ctypes.c_int),
     ]
 
 
@@ -107,6 +119,8 @@ SIGNATURES = {
     "nb_direct_f64": (_i, [_vp, _vp, _i, _vp]),
     "nb_total_energy_tree": (_i, [_vp, ctypes.c_float, _vp]),
     "nb_autotune": (_i, [_vp, _i, ctypes.c_float, ctypes.POINTER(NbTuneResult), _cp, _i]),
+    "nb_set_launch": (_i, [_vp, ctypes.POINTER(NbLaunch)]),
+    "nb_get_launch": (_i, [_vp, ctypes.POINTER(NbLaunch)]),
     "nb_compute_forces_noalloc": (_i, [_vp, _i, _pp, _vp, _vp]),
 }
 

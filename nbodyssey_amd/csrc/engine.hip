@@ -147,7 +147,8 @@ BhScratch carve_bh_scratch(void* base, int n, size_t* bytes) {
 }
 
 void enqueue_bh_force(hipStream_t stream, const BhScratch& b, float4* pos, float4* vel,
-                      float4* acc, int n, const Params& p, bool step, PhaseMarks* marks) {
+                      float4* acc, int n, const Params& p, bool step, PhaseMarks* marks,
+                      int walk_stack) {
   const float half = 0.5f * p.dt;
   auto mark = [&](int k) {
     if (marks) marks->mark(k);
@@ -183,6 +184,7 @@ void enqueue_bh_force(hipStream_t stream, const BhScratch& b, float4* pos, float
   w.error = &b.state->error;
   w.compensated = p.precision == 1;
   w.half_far = p.precision == 2;
+  w.stack = walk_stack;
   launch_walk(stream, p.traverse, p.group_width, w);
   mark(5);
 }
@@ -268,7 +270,7 @@ class Engine {
       } marks;
       marks.e = this;
       marks.ce = &ce;
-      enqueue_bh_force(stream, scratch_view(), pos, vel, acc, n, p, step, &marks);
+      enqueue_bh_force(stream, scratch_view(), pos, vel, acc, n, p, step, &marks, walk_stack);
       tree_valid = true;
     } else if (bh) {
       // n < 2: zero force (forces_bh.cu:466-469)
@@ -279,7 +281,7 @@ class Engine {
       if (step) launch_kick_drift(stream, pos, vel, acc, n, half, p.dt, false, nullptr);
       const int split = ap_split > 0 ? ap_split : allpairs_split(n);
       launch_allpairs(stream, pos, n, eps2, p.G, acc, ap_partial.p, split, p.precision == 1,
-                      step ? vel : nullptr, half);
+                      step ? vel : nullptr, half, ap_tile);
       mark(ce, 1);
     }
     if (timed) end_call(ce);
@@ -328,6 +330,7 @@ class Engine {
     w.error = &state.p->error;
     w.compensated = p.precision == 1;
     w.half_far = p.precision == 2;
+    w.stack = walk_stack;
     return w;
   }
 
@@ -503,11 +506,13 @@ class Engine {
   // fixed targets) stays within max_err; the particle state is saved and
   // restored, so tuning does not advance the simulation.
   //   Barnes-Hut: walk group width {64, 32, 16} x key bits {30, 63} x graph
-  //               replay {off, on};
-  //   all-pairs:  source split {auto, 2x, 1/2, 1} x graph {off, on};
+  //               replay {off, on}, and at width 64 also the walk's LDS stack
+  //               depth {512, 768, 1024, 2048} per wave;
+  //   all-pairs:  source split {auto, 2x, 1/2, 1} x LDS tile width {256, 512,
+  //               1024} x graph {off, on};
   //   ensemble:   persistent-walk grid {resident, 3/4, 1/2} x graph {off, on}.
   struct Candidate {
-    int group_width, key_bits, graph, split, grid;
+    int group_width, key_bits, graph, split, grid, tile, stack;
     double ms, err;
   };
   std::vector<Candidate> autotune(int steps, float max_err) {
@@ -516,7 +521,7 @@ class Engine {
     sync();
     const Params p0 = p;
     const bool graph0 = graph_mode;
-    const int split0 = ap_split, grid0 = walk_grid;
+    const int split0 = ap_split, grid0 = walk_grid, tile0 = ap_tile, stack0 = walk_stack;
     DevBuf<float4> save;
     save.alloc((size_t)3 * n);
     NB_HIP(hipMemcpyAsync(save.p, pos, n * sizeof(float4), hipMemcpyDeviceToDevice, stream));
@@ -543,23 +548,30 @@ class Engine {
     if (nsys > 0) {
       const int full = walk_items_resident();
       for (int g : {full, full * 3 / 4, full / 2})
-        for (int gr = 0; gr < 2; ++gr) cands.push_back({64, 30, gr, 0, std::max(g, 1), 0, 0});
+        for (int gr = 0; gr < 2; ++gr)
+          cands.push_back({64, 30, gr, 0, std::max(g, 1), 0, 0, 0, 0});
     } else if (p.force == 1 && p.traverse == 0) {
       // the per-target walk keeps its own approximation: only key width and
       // graph replay are tuned (group widths would change the walk's result)
       for (int kb : {30, 63})
-        for (int gr = 0; gr < 2; ++gr) cands.push_back({p.group_width, kb, gr, 0, 0, 0, 0});
+        for (int gr = 0; gr < 2; ++gr) cands.push_back({p.group_width, kb, gr, 0, 0, 0, 0, 0, 0});
     } else if (p.force == 1) {
       for (int kb : {30, 63})
         for (int w : {64, 32, 16})
-          for (int gr = 0; gr < 2; ++gr) cands.push_back({w, kb, gr, 0, 0, 0, 0});
+          for (int gr = 0; gr < 2; ++gr) {
+            cands.push_back({w, kb, gr, 0, 0, 0, 1024, 0, 0});
+            if (w == 64)
+              for (int st : {512, 768, 2048}) cands.push_back({w, kb, gr, 0, 0, 0, st, 0, 0});
+          }
     } else {
       const int a = allpairs_split(n);
       std::vector<int> splits = {a, std::min(a * 2, 64), std::max(a / 2, 1), 1};
       std::sort(splits.begin(), splits.end());
       splits.erase(std::unique(splits.begin(), splits.end()), splits.end());
       for (int sp : splits)
-        for (int gr = 0; gr < 2; ++gr) cands.push_back({p.group_width, p.key_bits, gr, sp, 0, 0, 0});
+        for (int tl : {256, 512, 1024})
+          for (int gr = 0; gr < 2; ++gr)
+            cands.push_back({p.group_width, p.key_bits, gr, sp, 0, tl, 0, 0, 0});
     }
     const bool timing0 = timing_on;
     hipEvent_t e0, e1;
@@ -575,6 +587,8 @@ class Engine {
         set_params(q);
         ap_split = c.split;
         walk_grid = c.grid;
+        ap_tile = c.tile;
+        walk_stack = c.stack;
         drop_graph();
         graph_mode = c.graph != 0;
         timing_on = false;
@@ -617,6 +631,8 @@ class Engine {
         set_params(p0);
         ap_split = split0;
         walk_grid = grid0;
+        ap_tile = tile0;
+        walk_stack = stack0;
         drop_graph();
         graph_mode = graph0;
         restore();
@@ -646,6 +662,8 @@ class Engine {
     set_params(q);
     ap_split = c.split;
     walk_grid = c.grid;
+    ap_tile = c.tile;
+    walk_stack = c.stack;
     drop_graph();
     graph_mode = c.graph != 0;
     restore();
@@ -894,6 +912,8 @@ class Engine {
   DevBuf<unsigned long long> lookback;
   int ap_split = 0;    // all-pairs source split (0: automatic)
   int walk_grid = 0;   // persistent-walk workgroups (0: all resident)
+  int ap_tile = 0;     // all-pairs sources per LDS tile (0: 256)
+  int walk_stack = 0;  // wave64 group-walk LDS stack entries per wave (0: 1024)
   // ensemble (nsys > 0)
   int nsys = 0, nitems = 0, ntiles_e = 0;
   std::vector<int> h_off;
@@ -1069,15 +1089,18 @@ int nb_autotune(nb_engine* h, int steps, float max_median_err, nb_tune_result* o
       out->ms_per_step = c[0].ms;
       out->median_err = c[0].err;
       out->candidates = (int)c.size();
+      out->allpairs_tile = c[0].tile;
+      out->walk_stack = c[0].stack;
     }
     if (log && log_len > 0) {
       std::string t;
-      char line[160];
+      char line[200];
       for (size_t k = 0; k < c.size(); ++k) {
         std::snprintf(line, sizeof line,
-                      "%s width=%d keys=%d graph=%d split=%d grid=%d ms/step=%.4f median_err=%.3e\n",
+                      "%s width=%d keys=%d graph=%d split=%d tile=%d stack=%d grid=%d "
+                      "ms/step=%.4f median_err=%.3e\n",
                       k == 0 ? "chosen " : "tried  ", c[k].group_width, c[k].key_bits, c[k].graph,
-                      c[k].split, c[k].grid, c[k].ms, c[k].err);
+                      c[k].split, c[k].tile, c[k].stack, c[k].grid, c[k].ms, c[k].err);
         t += line;
       }
       std::snprintf(log, (size_t)log_len, "%s", t.c_str());
@@ -1263,6 +1286,40 @@ int nb_set_graph(nb_engine* h, int on) {
     need(h, "engine");
     h->e->drop_graph();
     h->e->graph_mode = on != 0;
+  });
+}
+
+int nb_set_launch(nb_engine* h, const nb_launch* l) {
+  return guarded([&] {
+    need(h, "engine");
+    need(l, "launch");
+    if (l->allpairs_split < 0 || l->allpairs_split > 64)
+      throw nb::Error(nb::kErrInvalid, "allpairs_split must be in [0, 64]");
+    if (l->allpairs_tile != 0 && l->allpairs_tile != 256 && l->allpairs_tile != 512 &&
+        l->allpairs_tile != 1024)
+      throw nb::Error(nb::kErrInvalid, "allpairs_tile must be 0, 256, 512 or 1024");
+    if (l->walk_stack != 0 && l->walk_stack != 512 && l->walk_stack != 768 &&
+        l->walk_stack != 1024 && l->walk_stack != 2048)
+      throw nb::Error(nb::kErrInvalid, "walk_stack must be 0, 512, 768, 1024 or 2048");
+    if (l->walk_grid < 0) throw nb::Error(nb::kErrInvalid, "walk_grid must be >= 0");
+    nb::Engine& e = *h->e;
+    e.drop_graph();
+    e.ap_split = l->allpairs_split;
+    e.ap_tile = l->allpairs_tile;
+    e.walk_stack = l->walk_stack;
+    e.walk_grid = l->walk_grid;
+  });
+}
+
+int nb_get_launch(const nb_engine* h, nb_launch* l) {
+  return guarded([&] {
+    need(h, "engine");
+    need(l, "launch");
+    const nb::Engine& e = *h->e;
+    l->allpairs_split = e.ap_split;
+    l->allpairs_tile = e.ap_tile;
+    l->walk_stack = e.walk_stack;
+    l->walk_grid = e.walk_grid;
   });
 }
 

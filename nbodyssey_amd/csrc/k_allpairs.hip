@@ -1,7 +1,9 @@
 // Tiled all-pairs gravity and the FP64 energy diagnostic.
 //
 // forces_naive (forces.cu:33-60): every target sums the softened pull of
-// every source, sources streamed through LDS in kBlock-wide float4 tiles.
+// every source, sources streamed through LDS in float4 tiles of TILE sources
+// (256, 512 or 1024: one barrier pair per tile; the plain FP32 sums are the
+// same bits at every tile width, the compensated ones Kahan-add per tile).
 // Differences in this design:
 //   * each thread owns kTargets targets (1: measured fastest on gfx950 --
 //     the loop is VALU-issue bound at ~13 VALU per interaction, and more
@@ -26,13 +28,14 @@ namespace {
 constexpr int kTargets = 1;
 constexpr int kSpan = kBlock * kTargets;   // targets per block
 
-template <bool COMP>
+template <bool COMP, int TILE>
 __global__ __launch_bounds__(kBlock) void allpairs_kernel(const float4* __restrict__ pos, int n,
                                                           int chunk, float eps2, float G,
                                                           float4* __restrict__ out,
                                                           float4* __restrict__ vel,
                                                           float half_dt, int direct) {
-  __shared__ float4 tile[kBlock];
+  static_assert(TILE % kBlock == 0, "a tile is whole workgroup loads");
+  __shared__ float4 tile[TILE];
   float bx[kTargets], by[kTargets], bz[kTargets];
   float sx[kTargets], sy[kTargets], sz[kTargets];
   float cx[kTargets], cy[kTargets], cz[kTargets];
@@ -47,9 +50,13 @@ __global__ __launch_bounds__(kBlock) void allpairs_kernel(const float4* __restri
   }
   const int s0 = blockIdx.y * chunk;
   const int s1 = min(n, s0 + chunk);
-  for (int t = s0; t < s1; t += kBlock) {
-    int j = t + threadIdx.x;
-    tile[threadIdx.x] = j < s1 ? pos[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int t = s0; t < s1; t += TILE) {
+#pragma unroll
+    for (int q = 0; q < TILE / kBlock; ++q) {
+      const int j = t + q * kBlock + threadIdx.x;
+      // massless padding: an exact +0 in every sum
+      tile[q * kBlock + threadIdx.x] = j < s1 ? pos[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
     __syncthreads();
     float px[kTargets], py[kTargets], pz[kTargets];
 #pragma unroll
@@ -58,7 +65,7 @@ __global__ __launch_bounds__(kBlock) void allpairs_kernel(const float4* __restri
       else { px[q] = sx[q]; py[q] = sy[q]; pz[q] = sz[q]; }
     }
 #pragma unroll 8
-    for (int k = 0; k < kBlock; ++k) {
+    for (int k = 0; k < TILE; ++k) {
       float4 src = tile[k];
 #pragma unroll
       for (int q = 0; q < kTargets; ++q)
@@ -284,9 +291,9 @@ int split_for(int blocks_x, int n) {
   return split < 1 ? 1 : split;
 }
 
-int chunk_for(int n, int split) {
+int chunk_for(int n, int split, int tile = kBlock) {
   int c = (n + split - 1) / split;
-  return ((c + kBlock - 1) / kBlock) * kBlock;
+  return ((c + tile - 1) / tile) * tile;
 }
 
 }  // namespace
@@ -300,20 +307,31 @@ int allpairs_split(int n) { return split_for((n + kSpan - 1) / kSpan, n); }
 
 void launch_allpairs(hipStream_t s, const float4* pos, int n, float eps2, float G, float4* acc,
                      float4* partial, int split, bool compensated, float4* vel_kick,
-                     float half_dt) {
+                     float half_dt, int tile) {
   if (n <= 0) return;
+  if (tile == 0) tile = kBlock;
+  if (tile != 256 && tile != 512 && tile != 1024)
+    throw Error(kErrInvalid, "all-pairs tile must be 256, 512 or 1024 sources");
+  if (split < 1) throw Error(kErrInvalid, "all-pairs split must be >= 1");
   int bx = (n + kSpan - 1) / kSpan;
-  int chunk = chunk_for(n, split);
+  int chunk = chunk_for(n, split, tile);
   split = (n + chunk - 1) / chunk;
   dim3 grid(bx, split);
   bool direct = split == 1;
   float4* out = direct ? acc : partial;
-  if (compensated)
-    allpairs_kernel<true><<<grid, kBlock, 0, s>>>(pos, n, chunk, eps2, G, out,
-                                                  direct ? vel_kick : nullptr, half_dt, direct);
-  else
-    allpairs_kernel<false><<<grid, kBlock, 0, s>>>(pos, n, chunk, eps2, G, out,
-                                                   direct ? vel_kick : nullptr, half_dt, direct);
+  float4* kick = direct ? vel_kick : nullptr;
+#define NB_AP(C, T) \
+  allpairs_kernel<C, T><<<grid, kBlock, 0, s>>>(pos, n, chunk, eps2, G, out, kick, half_dt, direct)
+  if (compensated) {
+    if (tile == 256) NB_AP(true, 256);
+    else if (tile == 512) NB_AP(true, 512);
+    else NB_AP(true, 1024);
+  } else {
+    if (tile == 256) NB_AP(false, 256);
+    else if (tile == 512) NB_AP(false, 512);
+    else NB_AP(false, 1024);
+  }
+#undef NB_AP
   NB_LAUNCH_CHECK();
   if (!direct) {
     int g = (n + kBlock - 1) / kBlock;

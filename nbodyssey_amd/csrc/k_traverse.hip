@@ -88,7 +88,7 @@ __device__ __forceinline__ void finish(int t, int n, float ax, float ay, float a
 // their (x, y, z, m) to an LDS interaction list that all 64 lanes evaluate in
 // blocks of 64 (LDS broadcast reads, pure FP32 VALU), optionally Kahan-adding
 // each block's partial sums (compensated mode).
-constexpr int kWaveStack = 1024;   // LDS stack entries per wave
+constexpr int kWaveStack = 1024;   // LDS stack entries per wave (default)
 // Per-target walk: 768 stack entries (3 KB, plus 3 KB of lane sets) let four
 // workgroups share a CU instead of three.  Its stack peaks near 260 entries at
 // theta 0.5 and 500 at theta 0.3 on 1M particles (1M step: 1.90 -> 1.74 ms at
@@ -406,7 +406,7 @@ struct WalkMode {
   static constexpr int l16 = half ? 2 * kHalfList : 1;     // half2 slots of the FP16 list
 };
 
-template <bool COUNT, int MODE, bool POT = false>
+template <bool COUNT, int MODE, bool POT = false, int STACK = kWaveStack>
 __device__ __forceinline__ void walk_group64(
     int t0, int count, int root, int slot, const float4* __restrict__ pos_sorted,
     const unsigned* __restrict__ perm, int n, const Node* __restrict__ nodes, float theta2,
@@ -415,7 +415,8 @@ __device__ __forceinline__ void walk_group64(
     float4* list, half2_t* l16 = nullptr, float* __restrict__ phi = nullptr) {
   constexpr bool COMP = WalkMode<MODE>::comp;
   constexpr bool HALF = WalkMode<MODE>::half;
-  constexpr int WSTACK = WalkMode<MODE>::stack;
+  constexpr int WSTACK = STACK;
+  static_assert(STACK >= kStackReserve + 128, "stack below the depth-first reserve");
   const int lane = threadIdx.x & 63;
   const bool live = lane < count;
   const int t = t0 + (live ? lane : count - 1);
@@ -692,23 +693,29 @@ __global__ __launch_bounds__(kBlock) void tree_energy_final_kernel(
 // visited / opened / accepted sets are exactly those of the gs-wide vote of
 // traverse_warp_kernel, node batches still use all 64 lanes, and only the
 // interaction-list evaluation idles the surplus lanes.
-template <bool COUNT, int MODE>
+//
+// STACK: LDS stack entries per wave.  The stack bounds occupancy (the kernel
+// needs 72 VGPRs, room for 7 waves per SIMD; with 1024 entries a workgroup
+// takes 24 KB of LDS and 6 fit a CU, with 768 entries 7); a shallower stack
+// reaches the single-pop reserve sooner (1M two-disk, theta 0.5: the deepest
+// group stack is 434 entries; theta 0.3: 756).
+template <bool COUNT, int MODE, int STACK>
 __global__ __launch_bounds__(kBlock) void walk_wave_kernel(
     const float4* __restrict__ pos_sorted, const unsigned* __restrict__ perm, int n,
     const Node* __restrict__ nodes, float theta2, float eps2, float G, float4* __restrict__ acc,
     float4* __restrict__ vel, float half_dt, int* __restrict__ counts, int* __restrict__ opens,
     int* overflow, int* error, int gs) {
   constexpr int WAVES = kBlock / 64;
-  __shared__ int s_stack[WAVES][WalkMode<MODE>::stack];
+  __shared__ int s_stack[WAVES][STACK];
   __shared__ float4 s_list[WAVES][kWaveList];
   __shared__ half2_t s_l16[WAVES][WalkMode<MODE>::l16];
   const int w = threadIdx.x >> 6;
   const int group = xcd_block() * WAVES + w;
   const int gbase = group * gs;
   if (gbase >= n) return;  // wave-uniform
-  walk_group64<COUNT, MODE>(gbase, min(gs, n - gbase), 0, group, pos_sorted, perm, n, nodes,
-                            theta2, eps2, G, acc, vel, half_dt, counts, opens, overflow, error,
-                            s_stack[w], s_list[w], s_l16[w]);
+  walk_group64<COUNT, MODE, false, STACK>(gbase, min(gs, n - gbase), 0, group, pos_sorted, perm,
+                                          n, nodes, theta2, eps2, G, acc, vel, half_dt, counts,
+                                          opens, overflow, error, s_stack[w], s_list[w], s_l16[w]);
 }
 
 // Group widths 32 and 16: S = 64 / width subgroups of consecutive targets
@@ -997,39 +1004,62 @@ __global__ __launch_bounds__(kBlock) void walk_items_kernel(
   }
 }
 
-template <bool COUNT>
-void walk_dispatch(hipStream_t s, int traverse, int width, const WalkArgs& a) {
-  const int g = (a.n + kBlock - 1) / kBlock;
 #define NB_WALK_ARGS                                                                          \
   a.pos_sorted, a.perm, a.n, a.nodes, a.theta2, a.eps2, a.G, a.acc, a.vel, a.half_dt, a.counts, \
       a.opens, a.overflow, a.error
+
+// The wave64 group walk in arithmetic MODE with the engine's stack depth.
+template <bool COUNT, int MODE>
+void launch_wave(hipStream_t s, int gw, int width, const WalkArgs& a) {
+  switch (a.stack) {
+    case 512:
+      walk_wave_kernel<COUNT, MODE, 512><<<gw, kBlock, 0, s>>>(NB_WALK_ARGS, width);
+      break;
+    case 768:
+      walk_wave_kernel<COUNT, MODE, 768><<<gw, kBlock, 0, s>>>(NB_WALK_ARGS, width);
+      break;
+    case 0:
+    case 1024:
+      walk_wave_kernel<COUNT, MODE, 1024><<<gw, kBlock, 0, s>>>(NB_WALK_ARGS, width);
+      break;
+    case 2048:
+      walk_wave_kernel<COUNT, MODE, 2048><<<gw, kBlock, 0, s>>>(NB_WALK_ARGS, width);
+      break;
+    default:
+      throw Error(kErrInvalid, "walk stack must be 512, 768, 1024 or 2048 entries");
+  }
+}
+
+template <bool COUNT>
+void walk_dispatch(hipStream_t s, int traverse, int width, const WalkArgs& a) {
+  const int g = (a.n + kBlock - 1) / kBlock;
   if (traverse == 0) {
     walk_target_kernel<COUNT><<<g, kBlock, 0, s>>>(NB_WALK_ARGS);
   } else if (width == 64 || width == 32 || width == 16) {
     const int groups = (a.n + width - 1) / width;
     const int gw = (groups + kBlock / 64 - 1) / (kBlock / 64);
     if (a.half_far && !COUNT)
-      walk_wave_kernel<COUNT, 2><<<gw, kBlock, 0, s>>>(NB_WALK_ARGS, width);
+      launch_wave<COUNT, 2>(s, gw, width, a);
     else if (a.compensated && !COUNT)
-      walk_wave_kernel<COUNT, 1><<<gw, kBlock, 0, s>>>(NB_WALK_ARGS, width);
+      launch_wave<COUNT, 1>(s, gw, width, a);
     else if (width == 32 && a.n >= kSplitMinN)
       walk_split_kernel<COUNT, 2><<<g, kBlock, 0, s>>>(NB_WALK_ARGS);
     else if (width == 16 && a.n >= kSplitMinN)
       walk_split_kernel<COUNT, 4><<<g, kBlock, 0, s>>>(NB_WALK_ARGS);
     else
-      walk_wave_kernel<COUNT, 0><<<gw, kBlock, 0, s>>>(NB_WALK_ARGS, width);
+      launch_wave<COUNT, 0>(s, gw, width, a);
   } else {
     throw Error(kErrInvalid, "group width must be 16, 32 or 64");
   }
-#undef NB_WALK_ARGS
   NB_LAUNCH_CHECK();
 }
+#undef NB_WALK_ARGS
 
 }  // namespace
 
 void warm_traverse() {
   hipFuncAttributes a;
-  NB_HIP(hipFuncGetAttributes(&a, reinterpret_cast<const void*>(walk_wave_kernel<false, 0>)));
+  NB_HIP(hipFuncGetAttributes(&a, reinterpret_cast<const void*>(walk_wave_kernel<false, 0, 1024>)));
 }
 
 // Resident workgroups of the persistent walk: CUs x occupancy, queried once.

@@ -142,9 +142,11 @@ void launch_fill_u32(hipStream_t s, unsigned* p, unsigned v, size_t count);
 
 // k_allpairs.hip
 int allpairs_split(int n);
+// split >= 1 source partitions; tile: sources per LDS tile, 256, 512 or 1024
+// (0: 256).
 void launch_allpairs(hipStream_t s, const float4* pos, int n, float eps2, float G,
                      float4* acc, float4* partial, int split, bool compensated,
-                     float4* vel_kick, float half_dt);
+                     float4* vel_kick, float half_dt, int tile = 0);
 void launch_energy(hipStream_t s, const float4* pos, const float4* vel, int n, float eps2,
                    float G, double* partial, int max_partial, double* out);
 int energy_partials(int n);
@@ -217,6 +219,8 @@ struct WalkArgs {
   int* error;         // set to 1 if a node id is out of range or the walk is too long
   bool compensated;   // Kahan-add each block of interactions (wave64 walk)
   bool half_far;      // FP16 far field for accepted cells (wave64 walk)
+  int stack;          // LDS stack entries per wave of the wave64 group walk:
+                      // 512, 768, 1024 or 2048 (0: 1024)
 };
 void launch_walk(hipStream_t s, int traverse, int group_width, const WalkArgs& a);
 // Energy from a potential walk at theta2 (< 1/3) on the built tree:
@@ -292,9 +296,11 @@ struct PhaseMarks {
 // order), enqueued on `stream` (forces_bh.cu:463-527).  step: the first kick
 // and the drift run first (fused with the bbox) and the closing kick rides in
 // the walk's epilogue on `vel`.  The sort state must have been reset (by
-// launch_sort_init, or by the previous pass's Karras kernel).
+// launch_sort_init, or by the previous pass's Karras kernel).  walk_stack:
+// LDS stack entries per wave of the wave64 group walk (WalkArgs::stack).
 void enqueue_bh_force(hipStream_t stream, const BhScratch& b, float4* pos, float4* vel,
-                      float4* acc, int n, const Params& p, bool step, PhaseMarks* marks);
+                      float4* acc, int n, const Params& p, bool step, PhaseMarks* marks,
+                      int walk_stack = 0);
 
 }  // namespace nb
 

@@ -123,6 +123,22 @@ class Simulation:
         check(self._L.nb_total_energy(self._h, ptr(out)))
         return out
 
+    def launch(self):
+        """The launch configuration (nb_get_launch) as a dict; 0 = automatic."""
+        l = _lib.NbLaunch()
+        check(self._L.nb_get_launch(self._h, ctypes.byref(l)))
+        return {k: getattr(l, k) for k, _ in _lib.NbLaunch._fields_}
+
+    def set_launch(self, **kw):
+        """Change fields of the launch configuration (nb_set_launch):
+        allpairs_split, allpairs_tile, walk_stack, walk_grid."""
+        cur = self.launch()
+        unknown = set(kw) - set(cur)
+        if unknown:
+            raise ValueError(f"unknown launch fields {sorted(unknown)}")
+        cur.update(kw)
+        check(self._L.nb_set_launch(self._h, ctypes.byref(_lib.NbLaunch(**cur))))
+
     def autotune(self, steps=5, max_median_err=0.01):
         """Pick the fastest launch configuration within the error bound
         (nb_autotune); returns (choice dict, per-candidate log lines)."""

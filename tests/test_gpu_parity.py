@@ -530,8 +530,7 @@ def test_autotune_ensemble():
 
 def test_far16_theta0_is_fp32():
     """With theta = 0 no cell is accepted, so the FP16 far field has nothing
-    to do: the result is the FP32 walk's up to summation order (the FP16 mode
-    keeps a smaller LDS stack, so its batches are cut differently)."""
+    to do: the result is the FP32 walk's up to FP32 summation order."""
     pos, _ = ic.two_disk_encounter(20000, seed=51, eps=0.02)
     kw = dict(eps=0.02, theta=0.0, force="bh", traverse="warp", group_width=64, key_bits=30)
     with engine.Simulation(pos, **kw) as a, engine.Simulation(pos, far16=True, **kw) as b:
