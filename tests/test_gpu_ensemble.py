@@ -130,11 +130,13 @@ def test_steps_and_energy_match_standalone():
 
 
 def test_ensemble_graph_matches_eager():
+    """Ten graph replays of the ensemble step, with eager work (downloads,
+    per-system energies) between them, equal the eager steps bitwise."""
     systems, _ = ic.ensemble_encounters(16, seed=200, n_min=2048, n_max=8192)
     kw = dict(dt=0.01, eps=EPS, theta=0.5)
     with engine.Ensemble(systems, **kw) as eager, engine.Ensemble(systems, **kw) as graph:
         graph.set_graph(True)
-        for r in range(3):
+        for r in range(10):
             print(f"ensemble graph replay {r}", flush=True)
             eager.step(1)
             graph.step(1)
@@ -184,3 +186,39 @@ def test_ensemble_far16():
         with engine.Simulation(pos, eps=EPS, theta=0.5, force="bh", traverse="warp",
                                group_width=64, key_bits=30, far16=True) as sim:
             assert np.array_equal(acc[off[s]:off[s + 1]], sim.compute_forces()), s
+
+
+def test_ensemble_256_systems_queue():
+    """The benchmark ensemble at full size (256 parabolic encounters, 2,048 to
+    65,536 particles each, 5.4M in all): the persistent walk drains the whole
+    item queue (every item records a non-empty walk), six graph-replayed
+    steps equal six eager steps bitwise, and the forces of the smallest, the
+    largest and six random systems equal their standalone walks bitwise."""
+    w = ic.WORKLOADS["ensemble-256-parabolic"]
+    systems, _ = ic.ensemble_encounters(256, seed=w["seed"])
+    print(f"ensemble 256: {sum(len(p) for p, _ in systems)} particles", flush=True)
+    kw = dict(dt=w["dt"], eps=w["eps"], theta=w["theta"])
+    sizes = np.array([len(p) for p, _ in systems])
+    pick = {int(np.argmin(sizes)), int(np.argmax(sizes))}
+    pick |= set(np.random.default_rng(9).choice(256, 6, replace=False).tolist())
+    with engine.Ensemble(systems, **kw) as ens:
+        acc = ens.compute_forces()
+        inter, opens = ens.traverse_counts()
+        assert len(inter) == len(ens.items()) and np.all(inter > 0) and np.all(opens > 0)
+        off = ens.offsets
+        for s in sorted(pick):
+            pos, _ = systems[s]
+            with engine.Simulation(pos, eps=w["eps"], theta=w["theta"], force="bh",
+                                   traverse="warp", group_width=64, key_bits=30) as sim:
+                a1 = sim.compute_forces()
+            o = int(off[s])
+            assert np.array_equal(acc[o:o + len(pos)], a1), f"system {s} forces differ"
+    with engine.Ensemble(systems, **kw) as eager, engine.Ensemble(systems, **kw) as graph:
+        graph.set_graph(True)
+        for r in range(6):
+            print(f"ensemble 256 step {r}", flush=True)
+            eager.step(1)
+            graph.step(1)
+            pe, ve = eager.download()
+            pg, vg = graph.download()
+            assert np.array_equal(pe, pg) and np.array_equal(ve, vg), f"step {r} differs"

@@ -343,8 +343,8 @@ def test_large_sort_properties_1m():
 
 @pytest.mark.parametrize("force", ["bh", "naive"])
 def test_graph_step_matches_eager(force):
-    """A captured-graph step is bitwise identical to the eager step, replay
-    after replay, with eager work (energy, downloads) between replays."""
+    """A captured-graph step is bitwise identical to the eager step over
+    twelve replays, with eager work (energy, downloads) between replays."""
     pos, vel = ic.two_disk_encounter(70001, seed=12, eps=0.02)
     kw = dict(dt=0.005, eps=0.02, theta=0.5, force=force, traverse="warp", group_width=64,
               key_bits=30)
@@ -352,7 +352,7 @@ def test_graph_step_matches_eager(force):
         graph.set_graph(True)
         eager.compute_forces()
         graph.compute_forces()
-        for r in range(4):
+        for r in range(12):
             print(f"graph replay {r} force={force}", flush=True)
             eager.step(1)
             graph.step(1)
