@@ -519,6 +519,7 @@ def bh_main(args, pos, vel, eps, dt, what, metric, data, energy_theta=None, cpu_
                             key_bits=args.key_bits, far16=args.far16)
     if args.walk_stack:
         sim.set_launch(walk_stack=args.walk_stack)
+    walk_stack = sim.launch()["walk_stack"] or 1024
     if args.graph:
         sim.set_graph(True)
     sim.compute_forces()              # a(x0)
@@ -606,7 +607,7 @@ def bh_main(args, pos, vel, eps, dt, what, metric, data, energy_theta=None, cpu_
                                f"walk={args.traverse}{'' if width == 1 else width}",
                    "n": n, "theta": args.theta, "key_bits": args.key_bits,
                    "traverse": args.traverse, "group_width": width,
-                   "walk_stack": sim.launch()["walk_stack"] or 1024},
+                   "walk_stack": walk_stack},
         "phase_ms": {k: phase[i] / max(calls, 1) for i, k in enumerate(names)},
         "build_roofline_hbm": build,
         "step_roofline": {"t_roof_ms": t_roof_ms, "t_meas_ms": ms_per_step,

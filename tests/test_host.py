@@ -224,3 +224,13 @@ def test_generate_ic_defaults_shape():
     pos, vel = ic.generate_ic_disks(particles=2000, seed=3)
     assert pos.shape == (2000, 4)
     np.testing.assert_allclose(pos[:, 3], 1.0 / 1000, rtol=1e-6)
+
+
+def test_bench_refuses_several_gpus():
+    """bench.py --gpus N > 1 exits with a message before touching a GPU."""
+    import sys
+    for argv in (["--gpus", "2"], ["--gpus", "8", "--steps", "1"]):
+        r = subprocess.run([sys.executable, "bench.py", *argv], capture_output=True, text=True,
+                           timeout=300, cwd=ROOT)
+        assert r.returncode == 2, argv
+        assert "one GPU" in r.stderr and r.stdout == ""
