@@ -800,6 +800,8 @@ class Engine {
     box_lo.alloc(c);
     box_hi.alloc(c);
     frontier.alloc((size_t)com_scratch_ints(n));
+    // the COM list counters start at zero (each build's COM passes reset them)
+    launch_fill_u32(stream, reinterpret_cast<unsigned*>(frontier.p), 0u, kComCounterInts);
     bh_partial.alloc(2 * kBboxBlocks);
     quant.alloc(1, true);
     state.alloc(1, true);

@@ -208,7 +208,7 @@ __global__ __launch_bounds__(kBlock) void ens_karras_kernel(
   nodes[i].other = j;
   up[left] = make_int4(i, right, j, 0);
   up[right] = make_int4(i, left, j, 0);
-  visit[i] = 0;
+  visit[i] = com_visit_word(first, last);
   if (i == 0) up[0] = make_int4(-1, -1, j, 0);
   // the node covering exactly one system is that system's subtree root
   const int s = (int)(ki >> 30);

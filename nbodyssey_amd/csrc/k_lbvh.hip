@@ -25,12 +25,19 @@
 // Karras build (forces_bh.cu:234-277): one thread per internal node.
 // COM (forces_bh.cu:281-331): leaves climb towards the root and the second
 // arrival at a node combines both children.  Pass 1 does this per workgroup
-// over 1024 consecutive leaves with LDS arrival counters, pass 2 per 65,536
-// leaves with 2-bit LDS counters, and pass 3 (one workgroup) finishes the few
-// nodes whose leaf range spans superblocks.  No
-// step needs device-wide release/acquire (which would write back / invalidate
-// an XCD's L2 on every climb step).  The combine is FP64 from FP32 children,
-// rounded once.
+// over 1024 consecutive leaves in LDS, pass 2 per 32,768 leaves and pass 3
+// (one workgroup) over the few nodes whose leaf range spans superblocks.  The
+// Karras kernel marks each internal node with the pass that owns it; passes
+// 2 and 3 stage their nodes in LDS with links to their parents, let every
+// frontier entry of the previous pass arrive at once, and then climb from the
+// nodes that became complete with four lanes per climb (lane c computes
+// component c) in a few waves.  Their critical path is the height of the
+// pass's part of the tree (about 24 and 28 levels at 1M two-disk leaves) at
+// ~0.35-0.5 us per level: the latency of one wave's step (one LDS read round
+// trip, the FP64 combine, the slot write and the arrival count).  No step
+// needs device-wide release/acquire (which would write back / invalidate an
+// XCD's L2 on every climb step), and no climb step waits for a global store.
+// The combine is FP64 from FP32 children, rounded once.
 #include "nb_device.hpp"
 #include "nb_internal.hpp"
 
@@ -413,7 +420,7 @@ constexpr int kKarrasStage = kBlock + 2 * kKarrasWin;
 template <typename K>
 struct KeyWindow {
   const K* __restrict__ keys;
-  const K* s;
+  const lds<K>* s;
   int wlo;
   __device__ __forceinline__ K operator()(int j) const {
     const unsigned k = (unsigned)(j - wlo);
@@ -451,7 +458,7 @@ __global__ __launch_bounds__(kBlock) void karras_kernel(const K* __restrict__ ke
     reset_sort_state(ss);
   }
   __syncthreads();
-  const KeyWindow<K> key{keys, s_key, wlo};
+  const KeyWindow<K> key{keys, (const lds<K>*)s_key, wlo};
   const int i = base + threadIdx.x;
   if (i >= n - 1) return;
   const K ki = key(i);
@@ -478,7 +485,7 @@ __global__ __launch_bounds__(kBlock) void karras_kernel(const K* __restrict__ ke
   // each child's upward record: parent, sibling, far end of the parent's range
   up[left] = make_int4(i, right, j, 0);
   up[right] = make_int4(i, left, j, 0);
-  visit[i] = 0;
+  visit[i] = com_visit_word(first, last);
   if (i == 0) up[0] = make_int4(-1, -1, j, 0);
 }
 
@@ -501,17 +508,47 @@ __device__ __forceinline__ NodeData load_data(const float4* __restrict__ pos_sor
   return NodeData{*pc, *pl, *ph};
 }
 
+// n / m for the three COM components, each the IEEE FP64 quotient: one
+// reciprocal of m, refined by two Newton steps, is shared by the three, and
+// each quotient is q0 = n r corrected once by the residual, q = q0 + (n - m q0) r
+// (Markstein's method; the same steps as the compiler's FP64 division when its
+// exponent scaling does nothing, which holds for all finite operands formed
+// from FP32 values: |n| < 2^257, 2^-149 <= |m| < 2^129, so every quotient and
+// residual stays far inside the normal FP64 range).  A zero, infinite or NaN
+// operand takes the plain division.
+__device__ __forceinline__ bool finite64(double x) {
+  return __builtin_fabs(x) <= 1.7976931348623157e308;
+}
+__device__ __forceinline__ bool nr_div_ok(double n, double m) {
+  return m != 0.0 && finite64(m) && finite64(n);
+}
+__device__ __forceinline__ double nr_recip(double m) {
+  double r = __builtin_amdgcn_rcp(m);
+  r = __builtin_fma(r, __builtin_fma(-m, r, 1.0), r);
+  return __builtin_fma(r, __builtin_fma(-m, r, 1.0), r);
+}
+__device__ __forceinline__ double nr_quot(double n, double m, double r) {
+  const double q0 = n * r;  // a zero n: q0 is the correctly signed zero
+  return n == 0.0 ? q0 : __builtin_fma(__builtin_fma(-m, q0, n), r, q0);
+}
+__device__ __forceinline__ double3 div3(double nx, double ny, double nz, double m) {
+  if (!(nr_div_ok(nx, m) && finite64(ny) && finite64(nz)))
+    return make_double3(nx / m, ny / m, nz / m);
+  const double r = nr_recip(m);
+  return make_double3(nr_quot(nx, m, r), nr_quot(ny, m, r), nr_quot(nz, m, r));
+}
+
 // Combine two children (forces_bh.cu:313-324): FP64 mass-weighted centre of
 // mass rounded once to FP32, AABB union, squared longest edge.  The formula is
 // symmetric in the two children (IEEE + and fmin/fmax commute), so the caller
 // may pass them in either order.
 __device__ __forceinline__ NodeData combine(const NodeData& a, const NodeData& b) {
   const double ma = (double)a.com.w, mb = (double)b.com.w, m = ma + mb;
+  const double3 q = div3((double)a.com.x * ma + (double)b.com.x * mb,
+                         (double)a.com.y * ma + (double)b.com.y * mb,
+                         (double)a.com.z * ma + (double)b.com.z * mb, m);
   NodeData r;
-  r.com.x = (float)(((double)a.com.x * ma + (double)b.com.x * mb) / m);
-  r.com.y = (float)(((double)a.com.y * ma + (double)b.com.y * mb) / m);
-  r.com.z = (float)(((double)a.com.z * ma + (double)b.com.z * mb) / m);
-  r.com.w = (float)m;
+  r.com = make_float4((float)q.x, (float)q.y, (float)q.z, (float)m);
   r.lo = make_float4(fminf(a.lo.x, b.lo.x), fminf(a.lo.y, b.lo.y), fminf(a.lo.z, b.lo.z), 0.f);
   r.hi = make_float4(fmaxf(a.hi.x, b.hi.x), fmaxf(a.hi.y, b.hi.y), fmaxf(a.hi.z, b.hi.z), 0.f);
   return r;
@@ -531,13 +568,11 @@ __device__ __forceinline__ void store_node(Node* nodes, float4* box_lo, float4* 
 // below the segment size; overflow is reported as an error), so the passes
 // need no global atomics: entries are collected in LDS and one count per
 // workgroup is written.
-constexpr int kComThreads = 1024;
-constexpr int kComLeaves = 1024;    // pass-1 leaves per workgroup (one per thread)
-constexpr int kComSuper = 65536;    // pass-2 leaves per workgroup
-constexpr int kSeg = 256;           // frontier entries per workgroup segment
-constexpr int kTopSegs = 4096;      // superblocks the top pass can take (n <= 2^28)
+constexpr int kComThreads = 1024;  // one thread per pass-1 leaf
+constexpr int kTopSegs = 8192;      // superblocks the top pass can take (n <= 2^28)
 
-__device__ __forceinline__ void seg_push(int* s_list, unsigned* s_cnt, int v, int* error) {
+__device__ __forceinline__ void seg_push(lds<int>* s_list, lds<unsigned>* s_cnt, int v,
+                                         int* error) {
   const unsigned k = __hip_atomic_fetch_add(s_cnt, 1u, __ATOMIC_RELAXED,
                                             __HIP_MEMORY_SCOPE_WORKGROUP);
   if (k < (unsigned)kSeg) s_list[k] = v;
@@ -669,9 +704,10 @@ __global__ __launch_bounds__(kComThreads) void com_local_kernel(
       if (c < kComLeaves) {
         box_lo[lo + c] = d.lo;
         box_hi[lo + c] = d.hi;
-        seg_push(s_front, &s_nfront, lo + c, error);
+        seg_push((lds<int>*)s_front, (lds<unsigned>*)&s_nfront, lo + c, error);
       } else {
-        seg_push(s_front, &s_nfront, nm1 + lo + c - kComLeaves, error);
+        seg_push((lds<int>*)s_front, (lds<unsigned>*)&s_nfront, nm1 + lo + c - kComLeaves,
+                 error);
       }
       return false;
     }
@@ -757,88 +793,496 @@ __device__ __forceinline__ int climb_status(int4 u, int nm1, int total, int lo, 
   return u.x < 0 ? 0 : bad ? 1 : out ? 2 : 3;
 }
 
-// Climb through global node data, shared by passes 2 and 3.  `counter(node)`
-// returns the previous arrival count.  The parent's upward record was written
-// by an earlier launch, so it is fetched together with the sibling's data:
-// one dependent global round trip per level.
-template <typename Counter>
-__device__ __forceinline__ void climb_global(const float4* __restrict__ pos_sorted, int n,
-                                             Node* nodes, const int4* __restrict__ up,
-                                             float4* box_lo, float4* box_hi, int child,
-                                             int lo, int hi, int* s_list, unsigned* s_cnt,
-                                             Counter counter, int* error) {
+// Climb shared by passes 2 and 3.  The two children of a node arrive through
+// its arrival word: each arriving child first puts its data (COM, AABB) in an
+// LDS slot of the workgroup and then ORs the slot's tag into the word; the
+// child that finds the word empty stops, and the one that finds a tag reads
+// its sibling from that slot, combines, stores the node for the walk (not
+// waited for) and climbs on.  A child's data thus reaches its sibling through
+// LDS: no level waits for a global store to complete or for a global load of
+// data written in this launch.  When the slot table is full, the child's data
+// is published in global memory instead (its stores completed before the
+// arrival, tag kTagGlobal), and the sibling reads it from there.
+//
+// `arrive(node, tag)` ORs tag into the node's word and returns the previous
+// tag (0: none).  Upward records were written by an earlier launch.
+struct ComSlots {
+  lds<float4>* com;
+  lds<float3>* lo;
+  lds<float3>* hi;
+  lds<unsigned>* used;  // slots handed out
+  unsigned cap;
+};
+constexpr unsigned kTagGlobal = 0xffffu;
+// A valid tree is at most 2^30 leaves deep along index bits plus the 63 key
+// bits: a climb longer than this is a malformed tree.
+constexpr int kMaxClimb = 128;
+
+template <typename Arrive>
+__device__ __forceinline__ void climb_exchange(const float4* __restrict__ pos_sorted, int n,
+                                               Node* nodes, const int4* __restrict__ up,
+                                               float4* box_lo, float4* box_hi, int child, int lo,
+                                               int hi, lds<int>* s_list, lds<unsigned>* s_cnt,
+                                               const ComSlots& slots, Arrive arrive, int* error) {
   const int nm1 = n - 1;
   const int total = 2 * n - 1;
   NodeData cur = load_data(pos_sorted, nm1, nodes, box_lo, box_hi, child);
   int4 u = up[child];
-  while (true) {
+  bool in_global = true;  // cur is also in global memory and visible (an earlier launch wrote it)
+  for (int level = 0;; ++level) {
     const int st = climb_status(u, nm1, total, lo, hi);
     if (st == 0) break;
-    if (st == 1) {
+    if (st == 1 || level > kMaxClimb) {
       *error = 1;
       break;
     }
-    if (st == 2) {
-      seg_push(s_list, s_cnt, child, error);
+    if (st == 2) {  // frontier entry of the next pass (none in the last pass)
+      if (s_list) seg_push(s_list, s_cnt, child, error);
+      else *error = 1;
       break;
     }
-    const int node = u.x;
-    const unsigned prev = counter(node);
-    if (prev != 1u) {
-      if (prev > 1u) *error = 1;
-      break;
-    }
+    const int node = u.x, sib = u.y;
     const int4 next = up[node];
-    cur = combine(cur, load_data(pos_sorted, nm1, nodes, box_lo, box_hi, u.y));
+    const unsigned slot = __hip_atomic_fetch_add(slots.used, 1u, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+    unsigned tag;
+    if (slot < slots.cap) {
+      slots.com[slot] = cur.com;
+      slots.lo[slot] = make_float3(cur.lo.x, cur.lo.y, cur.lo.z);
+      slots.hi[slot] = make_float3(cur.hi.x, cur.hi.y, cur.hi.z);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+      tag = slot + 1;
+    } else {
+      // table full: the sibling reads this child from global memory, so its
+      // node stores (if this launch made them) must have completed
+      if (!in_global) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      tag = kTagGlobal;
+    }
+    const unsigned prev = arrive(node, tag);
+    if (prev == 0) break;  // first arrival: the sibling combines
+    NodeData other;
+    if (prev == kTagGlobal) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      other = load_data(pos_sorted, nm1, nodes, box_lo, box_hi, sib);
+    } else if (prev - 1 < slots.cap) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+      const float3 l = slots.lo[prev - 1], h = slots.hi[prev - 1];
+      other = NodeData{slots.com[prev - 1], make_float4(l.x, l.y, l.z, 0.f),
+                       make_float4(h.x, h.y, h.z, 0.f)};
+    } else {  // more than two arrivals: a malformed tree
+      *error = 1;
+      break;
+    }
+    cur = combine(cur, other);
     store_node(nodes, box_lo, box_hi, node, cur);
+    in_global = false;
     child = node;
     u = next;
   }
 }
 
+// Staged climb of passes 2 and 3: every node of the pass is staged in LDS
+// with its id and a link to its parent (the parent's staged index and this
+// node's side there).  Node k has two child slots; a child arriving at k
+// writes its data into its own slot (side 0 for the smaller id of the two
+// children), then counts its arrival.  The climb runs in two phases:
+//  * arrive_from: every frontier entry of the previous pass arrives at its
+//    staged parent, all threads in parallel; a second arrival puts the node
+//    on the ready list;
+//  * climb_ready (after a barrier): a few waves, one per SIMD, take ready
+//    nodes from the list, combine the two slots, store the node and arrive
+//    at its parent; a second arrival climbs on in the same lane, a first one
+//    takes the next ready node.
+// A climb step is VALU-heavy (the FP64 combine) and each wave issues it for
+// all its lanes, so the serial part of the pass costs per level the issue of
+// the waves that run it: few full waves beat many sparse ones.
+struct Staged {
+  lds<float4>* com;  // 2 per node
+  lds<float3>* lo;
+  lds<float3>* hi;
+  lds<int>* id;      // per node
+  lds<int>* link;
+  lds<unsigned>* arrived;
+  lds<int>* ready;   // nodes whose two children have arrived
+  lds<unsigned>* nready;
+  lds<unsigned>* head;  // ready nodes taken
+};
+constexpr int kSideBit = 1 << 30;
+constexpr int kLinkRoot = 0x3fffffff;  // the node is the root
+constexpr int kLinkOut = 0x3ffffffe;   // the parent belongs to a later pass
+constexpr int kClimbWaves = 4;         // waves of climb_ready: one per SIMD
+
+// Links of the staged nodes: on entry link[k] holds the parent's id and
+// arrived[k] the node's side; `index(id)` must be ready.
+template <typename Index>
+__device__ __forceinline__ void link_staged(const Staged& st, int count, Index index) {
+  for (int k = threadIdx.x; k < count; k += blockDim.x) {
+    const int parent = st.link[k];
+    const int pk = parent < 0 ? kLinkRoot : index(parent);
+    st.link[k] = (pk < 0 ? kLinkOut : pk) | (st.arrived[k] ? kSideBit : 0);
+    st.arrived[k] = 0;
+  }
+}
+
+// `cur` arrives at staged node k on `side`; returns true for the second
+// arrival (the node is then complete in LDS).
+__device__ __forceinline__ bool arrive_staged(const Staged& st, const NodeData& cur, int k,
+                                              int side, int* error) {
+  const int mine = 2 * k + side;
+  st.com[mine] = cur.com;
+  st.lo[mine] = make_float3(cur.lo.x, cur.lo.y, cur.lo.z);
+  st.hi[mine] = make_float3(cur.hi.x, cur.hi.y, cur.hi.z);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  const unsigned prev = __hip_atomic_fetch_add(&st.arrived[k], 1u, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+  if (prev > 1u) *error = 1;  // a third arrival: a malformed tree
+  return prev == 1u;
+}
+
+// Phase 1 for frontier entry `child` of the previous pass: its data and
+// upward record come from global memory (earlier launch).
+template <typename Index>
+__device__ __forceinline__ void arrive_from(const float4* __restrict__ pos_sorted, int n,
+                                            Node* nodes, const int4* __restrict__ up,
+                                            float4* box_lo, float4* box_hi, int child,
+                                            const Staged& st, Index index, lds<int>* s_list,
+                                            lds<unsigned>* s_cnt, int* error) {
+  const int nm1 = n - 1;
+  const NodeData cur = load_data(pos_sorted, nm1, nodes, box_lo, box_hi, child);
+  const int4 u = up[child];
+  if (u.x < 0) return;  // the root
+  if (u.x >= nm1 || (unsigned)u.y >= (unsigned)(2 * n - 1)) {
+    *error = 1;
+    return;
+  }
+  const int k = index(u.x);
+  if (k < 0) {
+    if (s_list) seg_push(s_list, s_cnt, child, error);
+    else *error = 1;
+    return;
+  }
+  if (arrive_staged(st, cur, k, child < u.y ? 0 : 1, error)) {
+    const unsigned r = __hip_atomic_fetch_add(st.nready, 1u, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_WORKGROUP);
+    st.ready[r] = k;  // r < count: each node is completed once
+  }
+}
+
+// Phase 2, run by the first kClimbWaves waves after a barrier that follows
+// phase 1.  Lane L takes ready nodes L, L + W, L + 2W, ... (W climbing lanes)
+// as its climbs end: no queue atomics, and a lane with none left exits.
+// `count` staged nodes bound the steps of one lane.
+__device__ __forceinline__ void climb_ready(Node* nodes, float4* box_lo, float4* box_hi,
+                                            const Staged& st, int count, lds<int>* s_list,
+                                            lds<unsigned>* s_cnt, int* error) {
+  constexpr unsigned kLanes = kClimbWaves * 64;
+  const unsigned nready = *st.nready;
+  unsigned r = threadIdx.x;
+  if (r >= nready) return;
+  int k = st.ready[r];
+  for (int step = 0;; ++step) {
+    if (step > count) {  // more combines than nodes: a malformed tree
+      *error = 1;
+      break;
+    }
+    int link = st.link[k];
+    int node = st.id[k];
+    const float4 c0 = st.com[2 * k], c1 = st.com[2 * k + 1];
+    const float3 l0 = st.lo[2 * k], l1 = st.lo[2 * k + 1];
+    const float3 h0 = st.hi[2 * k], h1 = st.hi[2 * k + 1];
+    asm volatile("" : "+v"(link), "+v"(node));  // issued with the slot reads, not after the combine
+    const NodeData cur = combine(NodeData{c0, make_float4(l0.x, l0.y, l0.z, 0.f),
+                                          make_float4(h0.x, h0.y, h0.z, 0.f)},
+                                 NodeData{c1, make_float4(l1.x, l1.y, l1.z, 0.f),
+                                          make_float4(h1.x, h1.y, h1.z, 0.f)});
+    store_node(nodes, box_lo, box_hi, node, cur);
+    const int pk = link & (kSideBit - 1);
+    bool next = false;
+    if (pk == kLinkRoot) {
+    } else if (pk == kLinkOut) {  // frontier entry of the next pass (none in the last pass)
+      if (s_list) seg_push(s_list, s_cnt, node, error);
+      else *error = 1;
+    } else if (arrive_staged(st, cur, pk, (link & kSideBit) ? 1 : 0, error)) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+      k = pk;
+      next = true;
+    }
+    if (!next) {  // this climb ended: the lane's next ready node, if any
+      r += kLanes;
+      if (r >= nready) break;
+      k = st.ready[r];
+    }
+  }
+}
+
+// Phase 2 with four lanes per climb: lane c of a quad computes component c of
+// the node's COM (c = 3: the mass) and AABB, with the operations combine()
+// applies to that component, so a climb step issues about a third of the
+// instructions of a one-lane step (a lone wave issues one instruction per
+// four cycles: the step's instruction count is its latency).  Every decision
+// is uniform in the quad: lane 0 makes the arrival count after the whole wave
+// has completed its slot writes, and a quad permutation broadcasts it.
+template <int kPerm>  // DPP quad_perm control: 0x00 lane 0, 0x55 lane 1, 0xaa lane 2
+__device__ __forceinline__ float quad_bcast(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), kPerm, 0xf, 0xf, false));
+}
+__device__ __forceinline__ void climb_ready4(Node* nodes, float4* box_lo, float4* box_hi,
+                                             const Staged& st, int count, lds<int>* s_list,
+                                             lds<unsigned>* s_cnt, int* error) {
+  constexpr unsigned kQuads = kClimbWaves * 16;
+  const unsigned nready = *st.nready;
+  const int c = threadIdx.x & 3;
+  const int cc = c < 3 ? c : 2;  // the AABB component read (lane 3 repeats z)
+  unsigned r = threadIdx.x >> 2;
+  if (r >= nready) return;  // whole quads
+  int k = st.ready[r];
+  lds<float>* com = reinterpret_cast<lds<float>*>(st.com);
+  lds<float>* lo = reinterpret_cast<lds<float>*>(st.lo);
+  lds<float>* hi = reinterpret_cast<lds<float>*>(st.hi);
+  // Rare cases (non-finite operands, the root, a frontier parent, a malformed
+  // tree) are handled under wave-uniform branches, so the common step has no
+  // divergent control flow besides the end of a climb.
+  for (int step = 0;; ++step) {
+    if (step > count) {  // more combines than nodes: a malformed tree
+      if (c == 0) *error = 1;
+      break;
+    }
+    int link = st.link[k];
+    int node = st.id[k];
+    float x0 = com[8 * k + c], x1 = com[8 * k + 4 + c];
+    float m0 = com[8 * k + 3], m1 = com[8 * k + 7];
+    float l0 = lo[6 * k + cc], l1 = lo[6 * k + 3 + cc];
+    float h0 = hi[6 * k + cc], h1 = hi[6 * k + 3 + cc];
+    int nk = st.ready[min(r + kQuads, nready - 1)];  // the next ready node, if this climb ends
+    // all reads in flight together: one LDS round trip
+    asm volatile("" : "+v"(link), "+v"(node), "+v"(x0), "+v"(x1), "+v"(m0), "+v"(m1), "+v"(l0),
+                 "+v"(l1), "+v"(h0), "+v"(h1), "+v"(nk));
+    const double ma = (double)m0, mb = (double)m1, m = ma + mb;
+    const double num = (double)x0 * ma + (double)x1 * mb;
+    const bool ok = nr_div_ok(num, m);
+    double quot = nr_quot(num, m, nr_recip(m));
+    if (__builtin_amdgcn_ballot_w64(!ok)) quot = ok ? quot : num / m;
+    const float q = c == 3 ? (float)m : (float)quot;
+    const float lmin = fminf(l0, l1), hmax = fmaxf(h0, h1);
+    const float edge = hmax - lmin;
+    const float sz = fmaxf(quad_bcast<0x00>(edge),
+                           fmaxf(quad_bcast<0x55>(edge), quad_bcast<0xaa>(edge)));
+    reinterpret_cast<float*>(&nodes[node].com)[c] = q;
+    reinterpret_cast<float*>(box_lo + node)[c] = c == 3 ? 0.f : lmin;
+    reinterpret_cast<float*>(box_hi + node)[c] = c == 3 ? 0.f : hmax;
+    nodes[node].size2 = sz * sz;  // the same value from the four lanes
+    const int pk = link & (kSideBit - 1);
+    const bool inner = pk < kLinkOut;  // the parent is staged in this pass
+    int prev = 0;
+    if (inner) {
+      const int mine = 2 * pk + ((link & kSideBit) ? 1 : 0);
+      com[4 * mine + c] = q;
+      lo[3 * mine + cc] = lmin;  // lane 3 writes lane 2's value to the same word
+      hi[3 * mine + cc] = hmax;
+      // the wave's slot writes complete before lane 0 counts the arrival
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+      prev = (int)__hip_atomic_fetch_add(&st.arrived[pk], c == 0 ? 1u : 0u, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+      prev = __builtin_amdgcn_mov_dpp(prev, 0x00, 0xf, 0xf, false);  // lane 0's count
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    }
+    if (__builtin_amdgcn_ballot_w64(!inner || prev > 1) && c == 0) {
+      if (inner && prev > 1) {  // a third arrival: a malformed tree
+        *error = 1;
+      } else if (!inner && pk == kLinkOut) {  // frontier entry of the next pass (not in the last)
+        if (s_list) seg_push(s_list, s_cnt, node, error);
+        else *error = 1;
+      }
+    }
+    if (inner && prev == 1) {
+      k = pk;
+    } else {  // this climb ended: the quad's next ready node, if any
+      r += kQuads;
+      if (r >= nready) break;
+      k = nk;
+    }
+  }
+}
+
+// Pass-3 node list, filled by pass 2 (one reservation per workgroup) and
+// read and reset by pass 3.
+struct TopList {
+  unsigned* count;
+  int* list;
+};
+
 // Pass 2: workgroup s owns the superblock of leaves [s*S, s*S + S): it climbs
-// from the frontier segments of its pass-1 workgroups with 2-bit arrival
-// counters in LDS (a node arrives at most twice, so adding 1 << 2k never
-// carries into the next field), indexed by node - lo.
+// from the frontier segments of its pass-1 workgroups.  Its own nodes are the
+// internal ids of the superblock whose visit word says pass 2 (an internal
+// node's id lies in its leaf range); it finds them with one coalesced read of
+// the 32,768 words, stages them (at most kMidCap; usually about 350) with a
+// 16-bit id -> index table, and lists the pass-3 ids it sees for pass 3.  A
+// superblock with more pass-2 nodes climbs with 16-bit arrival words in LDS
+// and exchange slots instead (climb_exchange).
+constexpr int kMidCap = 640;
+constexpr int kMidSlots = 1024;
+constexpr size_t kSlotBytes = 16 + 12 + 12;  // COM float4, lo float3, hi float3
+constexpr size_t kMidStagedBytes = 2 * kMidCap * kSlotBytes + 4 * kMidCap * 4 + kComSuper * 2;
+constexpr size_t kMidUnstagedBytes = kMidSlots * kSlotBytes + kComSuper / 2 * 4;
+constexpr size_t kMidBytes = kMidStagedBytes > kMidUnstagedBytes ? kMidStagedBytes
+                                                                 : kMidUnstagedBytes;
+constexpr int kMidTops = 256;  // pass-3 ids a superblock collects before listing them
 __global__ __launch_bounds__(kComThreads) void com_mid_kernel(
     const float4* __restrict__ pos_sorted, int n, Node* nodes, const int4* __restrict__ up,
-    float4* box_lo, float4* box_hi, const int* __restrict__ seg1,
-    const unsigned* __restrict__ cnt1, int nblocks1, int* seg2, unsigned* cnt2, int* error) {
-  __shared__ unsigned s_visit[kComSuper / 16];
+    const unsigned* __restrict__ visit, float4* box_lo, float4* box_hi,
+    const int* __restrict__ seg1, const unsigned* __restrict__ cnt1, int nblocks1, int* seg2,
+    unsigned* cnt2, TopList top, int* error) {
+  __shared__ __align__(16) unsigned char s_raw[kMidBytes];
   __shared__ int s_front[kSeg];
-  __shared__ unsigned s_nfront;
-  for (int k = threadIdx.x; k < kComSuper / 16; k += blockDim.x) s_visit[k] = 0;
-  if (threadIdx.x == 0) s_nfront = 0;
-  __syncthreads();
+  __shared__ int s_tops[kMidTops];
+  __shared__ unsigned s_nfront, s_used, s_nmid, s_ntop, s_topbase, s_nready, s_head;
+  const int nm1 = n - 1;
   const int lo = blockIdx.x * kComSuper;
   const int hi = min(n - 1, lo + kComSuper - 1);
+  // staged layout
+  lds<unsigned char>* raw = (lds<unsigned char>*)s_raw;
+  Staged st;
+  st.com = reinterpret_cast<lds<float4>*>(raw);
+  st.lo = reinterpret_cast<lds<float3>*>(st.com + 2 * kMidCap);
+  st.hi = st.lo + 2 * kMidCap;
+  st.id = reinterpret_cast<lds<int>*>(st.hi + 2 * kMidCap);
+  st.link = st.id + kMidCap;
+  st.arrived = reinterpret_cast<lds<unsigned>*>(st.link + kMidCap);
+  st.ready = reinterpret_cast<lds<int>*>(st.arrived + kMidCap);
+  st.nready = (lds<unsigned>*)&s_nready;
+  st.head = (lds<unsigned>*)&s_head;
+  lds<unsigned short>* s_index = reinterpret_cast<lds<unsigned short>*>(st.ready + kMidCap);
+  // unstaged layout
+  const ComSlots slots{reinterpret_cast<lds<float4>*>(raw),
+                       reinterpret_cast<lds<float3>*>(raw + kMidSlots * 16),
+                       reinterpret_cast<lds<float3>*>(raw + kMidSlots * 28),
+                       (lds<unsigned>*)&s_used, (unsigned)kMidSlots};
+  lds<unsigned>* s_arrive = reinterpret_cast<lds<unsigned>*>(raw + kMidSlots * kSlotBytes);
+
+  for (int k = threadIdx.x; k < kComSuper / 2; k += blockDim.x)
+    reinterpret_cast<lds<unsigned>*>(s_index)[k] = 0;
+  if (threadIdx.x == 0) {
+    s_nfront = 0;
+    s_used = 0;
+    s_nmid = 0;
+    s_ntop = 0;
+    s_nready = 0;
+    s_head = 0;
+  }
+  __syncthreads();
+  // this superblock's pass-2 nodes (into st.id) and pass-3 ids; every load
+  // is issued before the first is used, and a block-wide scan of the
+  // per-thread counts places them (no atomic per found node)
+  constexpr int kScan = kComSuper / kComThreads;
+  static_assert(kScan <= 32, "one 32-bit mask per thread");
+  const int last = min(hi, nm1 - 1);
+  unsigned word[kScan];
+#pragma unroll
+  for (int r = 0; r < kScan; ++r)  // unconditional (clamped) loads, so none waits for a branch
+    word[r] = visit[min(lo + r * kComThreads + (int)threadIdx.x, last)];
+  unsigned mid_mask = 0, top_mask = 0;
+#pragma unroll
+  for (int r = 0; r < kScan; ++r) {
+    const int i = lo + r * kComThreads + (int)threadIdx.x;
+    const unsigned pass = i <= last ? word[r] >> 30 : 0u;
+    mid_mask |= (pass == 1 ? 1u : 0u) << r;
+    top_mask |= (pass == 2 ? 1u : 0u) << r;
+  }
+  // both counts in one word: each is at most kComSuper < 2^16
+  static_assert(kComSuper < (1 << 16), "16-bit counts");
+  const unsigned packed = (unsigned)__popc(mid_mask) | ((unsigned)__popc(top_mask) << 16);
+  const unsigned incl = wave_incl_scan(packed);
+  __shared__ unsigned s_wsum[kComThreads / 64];
+  if ((threadIdx.x & 63) == 63) s_wsum[threadIdx.x >> 6] = incl;
+  __syncthreads();
+  unsigned base = incl - packed;
+  for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) base += s_wsum[w];
+  if (threadIdx.x == kComThreads - 1) {
+    s_nmid = (base + packed) & 0xffffu;
+    s_ntop = (base + packed) >> 16;
+  }
+  unsigned m1 = base & 0xffffu, m2 = base >> 16;
+#pragma unroll
+  for (int r = 0; r < kScan; ++r) {
+    const int i = lo + r * kComThreads + (int)threadIdx.x;
+    if ((mid_mask >> r) & 1u) {
+      if (m1 < (unsigned)kMidCap) st.id[m1] = i;
+      ++m1;
+    }
+    if ((top_mask >> r) & 1u) {
+      if (m2 < (unsigned)kMidTops) {
+        s_tops[m2] = i;
+      } else {  // many pass-3 ids in one superblock: list this one alone
+        const unsigned g = atomic_add_relaxed(top.count, 1u);
+        if (g < (unsigned)kTopList) top.list[g] = i;
+      }
+      ++m2;
+    }
+  }
+  __syncthreads();
+  const unsigned nmid = s_nmid;
+  const bool staged = nmid <= (unsigned)kMidCap;
+  const unsigned ntop = min(s_ntop, (unsigned)kMidTops);
+  if (threadIdx.x == 0 && ntop > 0) s_topbase = atomic_add_relaxed(top.count, ntop);
   constexpr int kPer = kComSuper / kComLeaves;  // pass-1 workgroups per superblock
   const int b0 = blockIdx.x * kPer;
-  auto counter = [&](int node) {
-    const int slot = node - lo;
-    const unsigned shift = 2u * (unsigned)(slot & 15);
-    const unsigned old = __hip_atomic_fetch_add(&s_visit[slot >> 4], 1u << shift,
-                                                __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-    return (old >> shift) & 3u;
-  };
   // compact the segments' entries so that entry j goes to thread j: one wave
   // loads the kPer counts together and scans them
-  static_assert(kPer == 64, "the segment scan is one wave wide");
+  static_assert(kPer <= 64, "the segment scan is one wave wide");
   __shared__ unsigned s_off[kPer + 1];
-  if (threadIdx.x < kPer) {
+  if (threadIdx.x < 64) {
     const int b = threadIdx.x;
-    const unsigned c = (b0 + b < nblocks1) ? cnt1[b0 + b] : 0u;
+    const unsigned c = (b < kPer && b0 + b < nblocks1) ? cnt1[b0 + b] : 0u;
     const unsigned incl = wave_incl_scan(c);
-    s_off[b + 1] = incl;
+    if (b < kPer) s_off[b + 1] = incl;
     if (b == 0) s_off[0] = 0;
   }
   __syncthreads();
+  if (ntop > 0 && threadIdx.x < ntop && s_topbase + threadIdx.x < (unsigned)kTopList)
+    top.list[s_topbase + threadIdx.x] = s_tops[threadIdx.x];
+  auto index = [&](int id) {
+    const int rel = id - lo;
+    return (rel >= 0 && rel < kComSuper) ? (int)s_index[rel] - 1 : -1;
+  };
+  if (staged) {
+    for (int k = threadIdx.x; k < (int)nmid; k += blockDim.x) {
+      const int id = st.id[k];
+      const int4 u = up[id];
+      st.link[k] = u.x;                  // parent id until link_staged
+      st.arrived[k] = id < u.y ? 0u : 1u;  // side until link_staged
+      s_index[id - lo] = (unsigned short)(k + 1);
+    }
+    __syncthreads();
+    link_staged(st, (int)nmid, index);
+  } else {
+    for (int k = threadIdx.x; k < kComSuper / 2; k += blockDim.x) s_arrive[k] = 0;
+  }
+  __syncthreads();
+  auto arrive = [&](int node, unsigned tag) {
+    const int k = node - lo;
+    const unsigned shift = 16u * (unsigned)(k & 1);
+    const unsigned old = __hip_atomic_fetch_or(&s_arrive[k >> 1], tag << shift,
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    return (old >> shift) & 0xffffu;
+  };
+  lds<int>* front = (lds<int>*)s_front;
+  lds<unsigned>* nfront = (lds<unsigned>*)&s_nfront;
   for (unsigned j = threadIdx.x; j < s_off[kPer]; j += blockDim.x) {
     int b = 0;
     for (int step = kPer / 2; step > 0; step >>= 1)  // last b with s_off[b] <= j
       if (s_off[b + step] <= j) b += step;
-    climb_global(pos_sorted, n, nodes, up, box_lo, box_hi, seg1[(b0 + b) * kSeg + (j - s_off[b])],
-                 lo, hi, s_front, &s_nfront, counter, error);
+    const int child = seg1[(b0 + b) * kSeg + (j - s_off[b])];
+    if (staged)
+      arrive_from(pos_sorted, n, nodes, up, box_lo, box_hi, child, st, index, front, nfront,
+                  error);
+    else
+      climb_exchange(pos_sorted, n, nodes, up, box_lo, box_hi, child, lo, hi, front, nfront,
+                     slots, arrive, error);
+  }
+  if (staged) {  // uniform: nmid is one value for the block
+    __syncthreads();
+    if (threadIdx.x < kClimbWaves * 64)
+      climb_ready4(nodes, box_lo, box_hi, st, (int)nmid, front, nfront, error);
   }
   __syncthreads();
   const unsigned cnt = min(s_nfront, (unsigned)kSeg);
@@ -847,18 +1291,57 @@ __global__ __launch_bounds__(kComThreads) void com_mid_kernel(
 }
 
 // Pass 3: one workgroup climbs from every superblock's frontier segment
-// through the few nodes that span superblocks, with arrival counters in global
-// memory (one workgroup, so workgroup scope holds; earlier passes are visible
-// across the kernel boundaries).
+// through the nodes that span superblocks (about 520 at 1M leaves), staged
+// from the list pass 2 made, with an LDS hash from id to index.  With more
+// than kTopCap listed nodes it climbs with arrival tags in the low bits of the
+// per-node `visit` words in global memory (written by the Karras kernel with
+// those bits clear; one workgroup, so workgroup scope holds) and exchange
+// slots instead.  Earlier passes are visible across the kernel boundaries.
+constexpr int kTopCap = 1024;
+constexpr int kTopHash = 2048;  // power of two
+constexpr int kTopSlots = 2048;
+constexpr size_t kTopStagedBytes = 2 * kTopCap * kSlotBytes + 4 * kTopCap * 4 + 2 * kTopHash * 4;
+constexpr size_t kTopUnstagedBytes = kTopSlots * kSlotBytes;
+constexpr size_t kTopBytes = kTopStagedBytes > kTopUnstagedBytes ? kTopStagedBytes
+                                                                 : kTopUnstagedBytes;
+__device__ __forceinline__ unsigned top_hash(int id) {
+  return ((unsigned)id * 2654435761u) >> (32 - 11);
+}
+static_assert(kTopHash == 1 << 11, "top_hash yields 11 bits");
+
 __global__ __launch_bounds__(kComThreads) void com_top_kernel(
     const float4* __restrict__ pos_sorted, int n, Node* nodes, const int4* __restrict__ up,
     unsigned* visit, float4* box_lo, float4* box_hi, const int* __restrict__ seg2,
-    const unsigned* __restrict__ cnt2, int nblocks2, int* error) {
-  __shared__ int s_dummy[1];
-  __shared__ unsigned s_dummy_cnt;
-  if (threadIdx.x == 0) s_dummy_cnt = 0;
-  __syncthreads();
-  auto counter = [&](int node) { return atomic_add_wg(&visit[node], 1u); };
+    const unsigned* __restrict__ cnt2, int nblocks2, TopList top, int* error) {
+  __shared__ __align__(16) unsigned char s_raw[kTopBytes];
+  __shared__ unsigned s_used, s_nready, s_head;
+  const int nm1 = n - 1;
+  const unsigned listed = *top.count;
+  const bool staged = listed <= (unsigned)kTopCap;
+  lds<unsigned char>* raw = (lds<unsigned char>*)s_raw;
+  Staged st;
+  st.com = reinterpret_cast<lds<float4>*>(raw);
+  st.lo = reinterpret_cast<lds<float3>*>(st.com + 2 * kTopCap);
+  st.hi = st.lo + 2 * kTopCap;
+  st.id = reinterpret_cast<lds<int>*>(st.hi + 2 * kTopCap);
+  st.link = st.id + kTopCap;
+  st.arrived = reinterpret_cast<lds<unsigned>*>(st.link + kTopCap);
+  st.ready = reinterpret_cast<lds<int>*>(st.arrived + kTopCap);
+  st.nready = (lds<unsigned>*)&s_nready;
+  st.head = (lds<unsigned>*)&s_head;
+  lds<int>* s_key = reinterpret_cast<lds<int>*>(st.ready + kTopCap);
+  lds<int>* s_val = s_key + kTopHash;
+  const ComSlots slots{reinterpret_cast<lds<float4>*>(raw),
+                       reinterpret_cast<lds<float3>*>(raw + kTopSlots * 16),
+                       reinterpret_cast<lds<float3>*>(raw + kTopSlots * 28),
+                       (lds<unsigned>*)&s_used, (unsigned)kTopSlots};
+  if (threadIdx.x == 0) {
+    s_used = 0;
+    s_nready = 0;
+    s_head = 0;
+  }
+  if (staged)
+    for (int k = threadIdx.x; k < kTopHash; k += blockDim.x) s_key[k] = -1;
   // compact the entries of all superblock segments so entry j goes to thread
   // j: each thread loads kTopPer consecutive counts, then a block-wide scan
   constexpr int kTopPer = kTopSegs / kComThreads;
@@ -883,16 +1366,75 @@ __global__ __launch_bounds__(kComThreads) void com_top_kernel(
     base += c[k];
   }
   if (threadIdx.x == kComThreads - 1) s_off[kTopSegs] = base;
+  if (staged) {
+    for (int k = threadIdx.x; k < (int)listed; k += blockDim.x) {
+      const int id = top.list[k];
+      if (id < 0 || id >= nm1) {  // only on a stale or corrupt list
+        *error = 1;
+        continue;
+      }
+      const int4 u = up[id];
+      st.id[k] = id;
+      st.link[k] = u.x;                  // parent id until link_staged
+      st.arrived[k] = id < u.y ? 0u : 1u;  // side until link_staged
+      unsigned h = top_hash(id);
+      bool placed = false;
+      for (int probe = 0; probe < kTopHash && !placed; ++probe) {
+        const int old = atomicCAS(&s_key[h], -1, id);
+        if (old == -1) {
+          s_val[h] = k;
+          placed = true;
+        } else if (old == id) {  // listed twice: a stale or corrupt list
+          *error = 1;
+          placed = true;
+        }
+        h = (h + 1) & (kTopHash - 1);
+      }
+    }
+  }
   __syncthreads();
+  auto index = [&](int id) {
+    unsigned h = top_hash(id);
+    for (int probe = 0; probe < kTopHash; ++probe) {
+      const int key = s_key[h];
+      if (key == id) return s_val[h];
+      if (key == -1) break;
+      h = (h + 1) & (kTopHash - 1);
+    }
+    return -1;
+  };
+  if (staged) {
+    link_staged(st, (int)listed, index);
+    __syncthreads();
+  }
+  auto arrive = [&](int node, unsigned tag) {
+    return __hip_atomic_fetch_or(&visit[node], tag, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP) & 0xffffu;
+  };
   const unsigned total_entries = s_off[nblocks2];
+  int top_step = 1;  // largest power of two <= nblocks2 (usually 32 segments, not kTopSegs)
+  while (top_step * 2 <= nblocks2) top_step *= 2;
   for (unsigned j = threadIdx.x; j < total_entries; j += blockDim.x) {
     int b = 0;
-    for (int step = kTopSegs / 2; step > 0; step >>= 1)  // last b with s_off[b] <= j
+    for (int step = top_step; step > 0; step >>= 1)  // last b with s_off[b] <= j
       if (b + step <= nblocks2 && s_off[b + step] <= j) b += step;
-    // the whole tree is in range here, so no climb stops at a frontier
-    climb_global(pos_sorted, n, nodes, up, box_lo, box_hi, seg2[b * kSeg + (j - s_off[b])], 0,
-                 n - 1, s_dummy, &s_dummy_cnt, counter, error);
+    const int child = seg2[b * kSeg + (j - s_off[b])];
+    // the whole tree is in range here: no climb stops at a frontier (a
+    // staged climb whose parent is not listed has a corrupt list)
+    if (staged)
+      arrive_from(pos_sorted, n, nodes, up, box_lo, box_hi, child, st, index, nullptr, nullptr,
+                  error);
+    else
+      climb_exchange(pos_sorted, n, nodes, up, box_lo, box_hi, child, 0, n - 1, nullptr,
+                     nullptr, slots, arrive, error);
   }
+  if (staged) {  // uniform: listed is one value for the block
+    __syncthreads();
+    if (threadIdx.x < kClimbWaves * 64)
+      climb_ready4(nodes, box_lo, box_hi, st, (int)listed, nullptr, nullptr, error);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *top.count = 0;  // for the next build
 }
 
 // Small inputs: one workgroup sorts every key in all passes.  A onesweep
@@ -1159,18 +1701,23 @@ void launch_karras(hipStream_t s, int key_bits, const void* keys, int n, Node* n
   NB_LAUNCH_CHECK();
 }
 
+namespace {
+int com_g1(int n) { return (n + kComLeaves - 1) / kComLeaves; }
+int com_g2(int n) { return (n + kComSuper - 1) / kComSuper; }
+}  // namespace
+
 int com_scratch_ints(int n) {
-  const int g1 = (n + kComLeaves - 1) / kComLeaves;
-  const int g2 = (n + kComSuper - 1) / kComSuper;
-  return (g1 + g2) * (kSeg + 1);
+  const int g1 = com_g1(n), g2 = com_g2(n);
+  return kComCounterInts + kTopList + (g1 + g2) * (kSeg + 1);
 }
 
 void launch_com(hipStream_t s, const float4* pos_sorted, int n, Node* nodes, const int4* up,
                 unsigned* visit, float4* box_lo, float4* box_hi, int* scratch, int* error) {
   if (n < 2) return;
-  const int g1 = (n + kComLeaves - 1) / kComLeaves;
-  const int g2 = (n + kComSuper - 1) / kComSuper;
-  int* seg1 = scratch;
+  const int g1 = com_g1(n), g2 = com_g2(n);
+  if (g2 > kTopSegs) throw Error(kErrInvalid, "particle count too large for the COM top pass");
+  const TopList top{reinterpret_cast<unsigned*>(scratch), scratch + kComCounterInts};
+  int* seg1 = top.list + kTopList;
   int* seg2 = seg1 + g1 * kSeg;
   unsigned* cnt1 = reinterpret_cast<unsigned*>(seg2 + g2 * kSeg);
   unsigned* cnt2 = cnt1 + g1;
@@ -1178,13 +1725,12 @@ void launch_com(hipStream_t s, const float4* pos_sorted, int n, Node* nodes, con
                                               cnt1, error);
   NB_LAUNCH_CHECK();
   if (g1 == 1) return;
-  com_mid_kernel<<<g2, kComThreads, 0, s>>>(pos_sorted, n, nodes, up, box_lo, box_hi, seg1, cnt1,
-                                            g1, seg2, cnt2, error);
+  com_mid_kernel<<<g2, kComThreads, 0, s>>>(pos_sorted, n, nodes, up, visit, box_lo, box_hi, seg1,
+                                            cnt1, g1, seg2, cnt2, top, error);
   NB_LAUNCH_CHECK();
   if (g2 == 1) return;
-  if (g2 > kTopSegs) throw Error(kErrInvalid, "particle count too large for the COM top pass");
   com_top_kernel<<<1, kComThreads, 0, s>>>(pos_sorted, n, nodes, up, visit, box_lo, box_hi, seg2,
-                                           cnt2, g2, error);
+                                           cnt2, g2, top, error);
   NB_LAUNCH_CHECK();
 }
 

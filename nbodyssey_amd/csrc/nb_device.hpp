@@ -116,6 +116,20 @@ __device__ __forceinline__ unsigned wave_incl_scan(unsigned v) {
 __device__ __forceinline__ unsigned atomic_add_agent(unsigned* p, unsigned v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
 }
+// An LDS (address space 3) object.  Pointers into __shared__ storage that
+// travel in structs or through selects can lose their address space, and
+// accesses through a generic pointer are flat instructions, which wait on
+// the global-memory counter as well; lds<T>* keeps them ds_* instructions.
+// A __shared__ array converts with a C-style cast: (lds<T>*)array.  The host
+// pass of a HIP compile only type-checks device code, so there it is T.
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+using lds = __attribute__((address_space(3))) T;
+#else
+template <typename T>
+using lds = T;
+#endif
+
 __device__ __forceinline__ unsigned atomic_add_wg(unsigned* p, unsigned v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
 }

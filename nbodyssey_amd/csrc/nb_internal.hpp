@@ -189,8 +189,29 @@ void launch_sort_u64(hipStream_t s, uint64_t* keys_a, uint64_t* keys_b, unsigned
 // not null.
 void launch_karras(hipStream_t s, int key_bits, const void* keys, int n, Node* nodes, int4* up,
                    unsigned* visit, SortState* ss);
-// COM scratch: per-workgroup frontier segments and counts (ints)
+// COM passes (k_lbvh.hip).  Pass 1 combines the nodes whose leaf range lies
+// in one block of kComLeaves leaves, pass 2 those inside one superblock of
+// kComSuper leaves, pass 3 the rest.  The Karras kernels store each internal
+// node's pass in the top two bits of its `visit` word (com_visit_word); pass 2
+// finds its nodes there and lists the pass-3 nodes it sees for pass 3.
+constexpr int kComLeaves = 1024;
+constexpr int kComSuper = 32768;
+constexpr int kSeg = 256;         // frontier entries per workgroup segment
+constexpr int kTopList = 4096;    // listed pass-3 nodes
+// Scratch of the COM passes (ints, com_scratch_ints(n) of them):
+//   topcnt[1]                  listed pass-3 nodes (zero between builds)
+//   toplist[kTopList]
+//   seg1[g1 * kSeg], seg2[g2 * kSeg], cnt1[g1], cnt2[g2]   frontier segments
+// with g1 = ceil(n / kComLeaves), g2 = ceil(n / kComSuper).
 int com_scratch_ints(int n);
+constexpr int kComCounterInts = 1;  // words at the start a new buffer must have zeroed
+// The `visit` word of internal node i of leaf range [first, last]: its COM
+// pass (0, 1 or 2) in bits 30-31, arrival bits clear.
+__device__ __forceinline__ unsigned com_visit_word(int first, int last) {
+  const unsigned pass = first / kComLeaves == last / kComLeaves ? 0u
+                        : first / kComSuper == last / kComSuper ? 1u : 2u;
+  return pass << 30;
+}
 void launch_com(hipStream_t s, const float4* pos_sorted, int n, Node* nodes, const int4* up,
                 unsigned* visit, float4* box_lo, float4* box_hi, int* scratch, int* error);
 

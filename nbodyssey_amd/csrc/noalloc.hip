@@ -38,8 +38,8 @@ bool g_ready[kMaxDevices] = {};
 BhScratch g_bufs[kMaxDevices];
 
 // This device's buffers; on first use, looks up the device's address of
-// `scratch`, checks that both layouts fit and clears the sort state and
-// look-back words.
+// `scratch`, checks that both layouts fit and clears the sort state, the COM
+// list counters and the look-back words.
 const BhScratch& device_buffers(int dev, hipStream_t s) {
   std::lock_guard<std::mutex> lock(g_mutex);
   if (g_ready[dev]) return g_bufs[dev];
@@ -53,6 +53,7 @@ const BhScratch& device_buffers(int dev, hipStream_t s) {
   if (need > kScratchBytes || fixed + naive > kScratchBytes)
     throw Error(kErrState, "static scratch too small for NB_NOALLOC_MAX_N particles");
   launch_fill_u32(s, reinterpret_cast<unsigned*>(b.state), 0u, sizeof(SortState) / 4);
+  launch_fill_u32(s, reinterpret_cast<unsigned*>(b.frontier), 0u, kComCounterInts);
   launch_fill_u32(s, reinterpret_cast<unsigned*>(b.lookback), 0u,
                   (size_t)kHistWords * b.max_tiles * 2);
   g_bufs[dev] = b;
