@@ -563,20 +563,40 @@ __device__ __forceinline__ void store_node(Node* nodes, float4* box_lo, float4* 
   box_hi[id] = d.hi;
 }
 
-// Frontier lists between the passes live in fixed per-workgroup segments
-// (a contiguous leaf range splits into at most ~2 x depth maximal subtrees, far
-// below the segment size; overflow is reported as an error), so the passes
-// need no global atomics: entries are collected in LDS and one count per
-// workgroup is written.
+// Hand-offs between the passes live in fixed per-workgroup segments of kSeg
+// slots in global memory, each with one count per workgroup, so the passes
+// need no global atomics and the next pass reads a segment in one round trip.
+// A contiguous leaf range meets at most ~2 x depth nodes that leave it, far
+// below kSeg; overflow is reported as an error.  Two kinds:
+//  * FrontEntry: a finished node or leaf whose parent belongs to a later pass,
+//    with its COM, AABB, link and id, so the next pass needs no further load;
+//  * node records (pass 1): the internal nodes of the block's id range that a
+//    later pass owns, as (id | pass << 30, link), so the next pass stages its
+//    nodes without scanning the pass words.
+// A link is parent id | side << 30 (side 0: the smaller id of the two
+// children; a leaf's id is n - 1 + its index), or -1 at the root.
 constexpr int kComThreads = 1024;  // one thread per pass-1 leaf
 constexpr int kTopSegs = 8192;      // superblocks the top pass can take (n <= 2^28)
+constexpr int kSideShift = 30;
+constexpr int kIdMask = (1 << kSideShift) - 1;
+struct FrontEntry {
+  float4 com;
+  float4 lo;  // w: the link, as int bits
+  float4 hi;  // w: the node's id, as int bits
+};
+__device__ __forceinline__ int make_link(int parent, int id, int sibling) {
+  return parent < 0 ? -1 : parent | ((id < sibling ? 0 : 1) << kSideShift);
+}
 
-__device__ __forceinline__ void seg_push(lds<int>* s_list, lds<unsigned>* s_cnt, int v,
-                                         int* error) {
+__device__ __forceinline__ void push_entry(FrontEntry* seg, lds<unsigned>* s_cnt,
+                                           const NodeData& d, int link, int id, int* error) {
   const unsigned k = __hip_atomic_fetch_add(s_cnt, 1u, __ATOMIC_RELAXED,
                                             __HIP_MEMORY_SCOPE_WORKGROUP);
-  if (k < (unsigned)kSeg) s_list[k] = v;
-  else *error = 1;
+  if (k < (unsigned)kSeg)
+    seg[k] = FrontEntry{d.com, make_float4(d.lo.x, d.lo.y, d.lo.z, __int_as_float(link)),
+                        make_float4(d.hi.x, d.hi.y, d.hi.z, __int_as_float(id))};
+  else
+    *error = 1;
 }
 
 // Pass 1: workgroup b owns the sorted leaves [b*L, b*L + L), one per thread.
@@ -600,12 +620,12 @@ __device__ __forceinline__ void seg_push(lds<int>* s_list, lds<unsigned>* s_cnt,
 // combines per round), which halves the rounds of a 1024-leaf block (13 ->
 // 7) without bringing back the idle lanes.  The combine is symmetric in the
 // two children, so the result does not depend on which child arrives second.
-// A child whose parent spans workgroups is a frontier entry (with its AABB,
-// which only the later passes need).
+// A child whose parent spans workgroups is a frontier entry; the internal
+// nodes of [lo, hi] that a later pass owns are listed as node records.
 //
 // Local slots: internal node lo+k is k, leaf lo+k is kComLeaves+k.  An upward
 // record in LDS is (parent slot, sibling slot), with parent = kUpRoot at the
-// root and kUpOut when the parent's range leaves the workgroup.
+// root, and (kUpOut, link) when the parent's range leaves the workgroup.
 constexpr int kUpRoot = -1;
 constexpr int kUpOut = -2;
 static_assert(kComLeaves == 1024, "list entries pack 11-bit child slots and 10-bit parents");
@@ -613,13 +633,14 @@ static_assert(kComLeaves == 1024, "list entries pack 11-bit child slots and 10-b
 // parent to the following round.
 constexpr int kClimb = 1;
 
-__device__ __forceinline__ int2 local_up(int4 u, int lo, int hi, int nm1, int total, int* error) {
+__device__ __forceinline__ int2 local_up(int4 u, int id, int lo, int hi, int nm1, int total,
+                                         int* error) {
   if (u.x < 0) return make_int2(kUpRoot, 0);
   if (u.x >= nm1 || (unsigned)u.y >= (unsigned)total) {
     *error = 1;
     return make_int2(kUpRoot, 0);
   }
-  if (min(u.x, u.z) < lo || max(u.x, u.z) > hi) return make_int2(kUpOut, 0);
+  if (min(u.x, u.z) < lo || max(u.x, u.z) > hi) return make_int2(kUpOut, make_link(u.x, id, u.y));
   // a local parent's children lie inside its range, so the sibling is local
   const int sib = u.y >= nm1 ? u.y - nm1 : u.y;
   if (sib < lo || sib > hi) {
@@ -631,7 +652,8 @@ __device__ __forceinline__ int2 local_up(int4 u, int lo, int hi, int nm1, int to
 
 __global__ __launch_bounds__(kComThreads) void com_local_kernel(
     const float4* __restrict__ pos_sorted, int n, Node* nodes, const int4* __restrict__ up,
-    float4* box_lo, float4* box_hi, int* seg1, unsigned* cnt1, int* error) {
+    const unsigned* __restrict__ visit, float4* box_lo, float4* box_hi, FrontEntry* seg1,
+    unsigned* cnt1, int2* rec1, unsigned* rcnt1, int* error) {
   __shared__ float4 s_leaf[kComLeaves];
   __shared__ float4 s_com[kComLeaves];
   __shared__ float3 s_lo[kComLeaves], s_hi[kComLeaves];
@@ -639,27 +661,46 @@ __global__ __launch_bounds__(kComThreads) void com_local_kernel(
   __shared__ unsigned s_visit[kComLeaves / 16];  // 2-bit arrival counters
   __shared__ unsigned s_list[2][kComLeaves];     // child | sibling << 11 | parent << 22
   __shared__ unsigned s_cnt[3];
-  __shared__ int s_front[kSeg];
-  __shared__ unsigned s_nfront;
+  __shared__ unsigned s_nfront, s_nrec;
   const int nm1 = n - 1;
   const int total = 2 * n - 1;
   const int lo = blockIdx.x * kComLeaves;
   const int hi = min(n - 1, lo + kComLeaves - 1);
   const int t = threadIdx.x;
+  FrontEntry* front = seg1 + (size_t)blockIdx.x * kSeg;
   if (t < kComLeaves / 16) s_visit[t] = 0;
   if (t < 3) s_cnt[t] = 0;
-  if (t == 0) s_nfront = 0;
+  if (t == 0) {
+    s_nfront = 0;
+    s_nrec = 0;
+  }
   int2 leaf_up = make_int2(kUpRoot, 0);
+  bool has_rec = false;  // this thread's internal node belongs to a later pass
+  int2 rec = make_int2(0, 0);
   if (lo + t <= hi) {
     const float4 p = pos_sorted[lo + t];
     const int4 ul = up[nm1 + lo + t];
     int4 un = make_int4(kUpRoot, 0, 0, 0);
-    if (lo + t < nm1) un = up[lo + t];
+    unsigned pass = 0;
+    if (lo + t < nm1) {
+      un = up[lo + t];
+      pass = visit[lo + t] >> 30;
+    }
     s_leaf[t] = p;
-    leaf_up = local_up(ul, lo, hi, nm1, total, error);
-    if (lo + t < nm1) s_up[t] = local_up(un, lo, hi, nm1, total, error);
+    leaf_up = local_up(ul, nm1 + lo + t, lo, hi, nm1, total, error);
+    if (lo + t < nm1) s_up[t] = local_up(un, lo + t, lo, hi, nm1, total, error);
+    if (pass != 0) {
+      has_rec = true;
+      rec = make_int2((int)((unsigned)(lo + t) | (pass << 30)), make_link(un.x, lo + t, un.y));
+    }
   }
   __syncthreads();
+  if (has_rec) {
+    const unsigned k = __hip_atomic_fetch_add(&s_nrec, 1u, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (k < (unsigned)kSeg) rec1[(size_t)blockIdx.x * kSeg + k] = rec;
+    else *error = 1;
+  }
 
   // A child's data is read from both layouts at once and then selected.  The
   // empty asm statements consume every value read, so the compiler can
@@ -701,14 +742,12 @@ __global__ __launch_bounds__(kComThreads) void com_local_kernel(
   auto report = [&](int c, const NodeData& d, int2 u) {
     if (u.x == kUpRoot) return false;
     if (u.x == kUpOut) {  // the parent spans workgroups: a frontier entry
-      if (c < kComLeaves) {
-        box_lo[lo + c] = d.lo;
-        box_hi[lo + c] = d.hi;
-        seg_push((lds<int>*)s_front, (lds<unsigned>*)&s_nfront, lo + c, error);
-      } else {
-        seg_push((lds<int>*)s_front, (lds<unsigned>*)&s_nfront, nm1 + lo + c - kComLeaves,
-                 error);
+      const int id = c < kComLeaves ? lo + c : nm1 + lo + c - kComLeaves;
+      if (c < kComLeaves) {  // for the passes' fallback climb, which reads global memory
+        box_lo[id] = d.lo;
+        box_hi[id] = d.hi;
       }
+      push_entry(front, (lds<unsigned>*)&s_nfront, d, u.y, id, error);
       return false;
     }
     const unsigned shift = 2u * (unsigned)(u.x & 15);
@@ -778,9 +817,10 @@ __global__ __launch_bounds__(kComThreads) void com_local_kernel(
       climb(slot, d, u, (r + 1) % 3, (r + 1) & 1);
     }
   }
-  const unsigned cnt = min(s_nfront, (unsigned)kSeg);
-  if (t < (int)cnt) seg1[blockIdx.x * kSeg + t] = s_front[t];
-  if (t == 0) cnt1[blockIdx.x] = cnt;
+  if (t == 0) {  // the last round's pushes came before its barrier
+    cnt1[blockIdx.x] = min(s_nfront, (unsigned)kSeg);
+    rcnt1[blockIdx.x] = min(s_nrec, (unsigned)kSeg);
+  }
 }
 
 // Where a climb goes from an upward record: 0 stop at the root, 1 malformed
@@ -822,10 +862,14 @@ template <typename Arrive>
 __device__ __forceinline__ void climb_exchange(const float4* __restrict__ pos_sorted, int n,
                                                Node* nodes, const int4* __restrict__ up,
                                                float4* box_lo, float4* box_hi, int child, int lo,
-                                               int hi, lds<int>* s_list, lds<unsigned>* s_cnt,
+                                               int hi, FrontEntry* seg, lds<unsigned>* s_cnt,
                                                const ComSlots& slots, Arrive arrive, int* error) {
   const int nm1 = n - 1;
   const int total = 2 * n - 1;
+  if ((unsigned)child >= (unsigned)total) {  // only from a corrupt segment
+    *error = 1;
+    return;
+  }
   NodeData cur = load_data(pos_sorted, nm1, nodes, box_lo, box_hi, child);
   int4 u = up[child];
   bool in_global = true;  // cur is also in global memory and visible (an earlier launch wrote it)
@@ -837,7 +881,7 @@ __device__ __forceinline__ void climb_exchange(const float4* __restrict__ pos_so
       break;
     }
     if (st == 2) {  // frontier entry of the next pass (none in the last pass)
-      if (s_list) seg_push(s_list, s_cnt, child, error);
+      if (seg) push_entry(seg, s_cnt, cur, make_link(u.x, child, u.y), child, error);
       else *error = 1;
       break;
     }
@@ -886,13 +930,13 @@ __device__ __forceinline__ void climb_exchange(const float4* __restrict__ pos_so
 // node's side there).  Node k has two child slots; a child arriving at k
 // writes its data into its own slot (side 0 for the smaller id of the two
 // children), then counts its arrival.  The climb runs in two phases:
-//  * arrive_from: every frontier entry of the previous pass arrives at its
+//  * arrive_entry: every frontier entry of the previous pass arrives at its
 //    staged parent, all threads in parallel; a second arrival puts the node
 //    on the ready list;
-//  * climb_ready (after a barrier): a few waves, one per SIMD, take ready
+//  * climb_ready4 (after a barrier): a few waves, one per SIMD, take ready
 //    nodes from the list, combine the two slots, store the node and arrive
-//    at its parent; a second arrival climbs on in the same lane, a first one
-//    takes the next ready node.
+//    at its parent; a second arrival climbs on, a first one takes the next
+//    ready node.
 // A climb step is VALU-heavy (the FP64 combine) and each wave issues it for
 // all its lanes, so the serial part of the pass costs per level the issue of
 // the waves that run it: few full waves beat many sparse ones.
@@ -905,21 +949,28 @@ struct Staged {
   lds<unsigned>* arrived;
   lds<int>* ready;   // nodes whose two children have arrived
   lds<unsigned>* nready;
-  lds<unsigned>* head;  // ready nodes taken
 };
-constexpr int kSideBit = 1 << 30;
-constexpr int kLinkRoot = 0x3fffffff;  // the node is the root
-constexpr int kLinkOut = 0x3ffffffe;   // the parent belongs to a later pass
-constexpr int kClimbWaves = 4;         // waves of climb_ready: one per SIMD
+// Staged link: parent index | side << 30, with two special parents: kLinkRoot
+// (the node is the root) and kOutBase + parent id (the parent belongs to a
+// later pass; ids are below 2^28 as n <= 2^28).
+constexpr int kSideBit = 1 << kSideShift;
+constexpr int kLinkRoot = 0x3fffffff;
+constexpr int kOutBase = 1 << 29;
+constexpr int kClimbWaves = 4;  // waves of climb_ready4: one per SIMD
 
-// Links of the staged nodes: on entry link[k] holds the parent's id and
-// arrived[k] the node's side; `index(id)` must be ready.
+// Links of the staged nodes: on entry link[k] holds the node's link from its
+// record (parent id | side << 30, -1 at the root); `index(id)` must be ready.
 template <typename Index>
 __device__ __forceinline__ void link_staged(const Staged& st, int count, Index index) {
   for (int k = threadIdx.x; k < count; k += blockDim.x) {
-    const int parent = st.link[k];
-    const int pk = parent < 0 ? kLinkRoot : index(parent);
-    st.link[k] = (pk < 0 ? kLinkOut : pk) | (st.arrived[k] ? kSideBit : 0);
+    const int link = st.link[k];
+    int pk = kLinkRoot;
+    if (link >= 0) {
+      const int parent = link & kIdMask;
+      const int ix = index(parent);
+      pk = ix >= 0 ? ix : kOutBase + parent;
+    }
+    st.link[k] = pk | (link >= 0 ? link & kSideBit : 0);
     st.arrived[k] = 0;
   }
 }
@@ -939,79 +990,29 @@ __device__ __forceinline__ bool arrive_staged(const Staged& st, const NodeData& 
   return prev == 1u;
 }
 
-// Phase 1 for frontier entry `child` of the previous pass: its data and
-// upward record come from global memory (earlier launch).
+// Phase 1 for a frontier entry of the previous pass.  An entry whose parent
+// is not staged here belongs to a later pass and is passed on to `out`.
 template <typename Index>
-__device__ __forceinline__ void arrive_from(const float4* __restrict__ pos_sorted, int n,
-                                            Node* nodes, const int4* __restrict__ up,
-                                            float4* box_lo, float4* box_hi, int child,
-                                            const Staged& st, Index index, lds<int>* s_list,
-                                            lds<unsigned>* s_cnt, int* error) {
-  const int nm1 = n - 1;
-  const NodeData cur = load_data(pos_sorted, nm1, nodes, box_lo, box_hi, child);
-  const int4 u = up[child];
-  if (u.x < 0) return;  // the root
-  if (u.x >= nm1 || (unsigned)u.y >= (unsigned)(2 * n - 1)) {
+__device__ __forceinline__ void arrive_entry(const FrontEntry& e, const Staged& st, Index index,
+                                             FrontEntry* out, lds<unsigned>* s_cnt,
+                                             int* error) {
+  const int link = __float_as_int(e.lo.w);
+  const NodeData d{e.com, make_float4(e.lo.x, e.lo.y, e.lo.z, 0.f),
+                   make_float4(e.hi.x, e.hi.y, e.hi.z, 0.f)};
+  if (link < 0) {  // the root is never an entry
     *error = 1;
     return;
   }
-  const int k = index(u.x);
+  const int k = index(link & kIdMask);
   if (k < 0) {
-    if (s_list) seg_push(s_list, s_cnt, child, error);
+    if (out) push_entry(out, s_cnt, d, link, __float_as_int(e.hi.w), error);
     else *error = 1;
     return;
   }
-  if (arrive_staged(st, cur, k, child < u.y ? 0 : 1, error)) {
+  if (arrive_staged(st, d, k, (link >> kSideShift) & 1, error)) {
     const unsigned r = __hip_atomic_fetch_add(st.nready, 1u, __ATOMIC_RELAXED,
                                               __HIP_MEMORY_SCOPE_WORKGROUP);
     st.ready[r] = k;  // r < count: each node is completed once
-  }
-}
-
-// Phase 2, run by the first kClimbWaves waves after a barrier that follows
-// phase 1.  Lane L takes ready nodes L, L + W, L + 2W, ... (W climbing lanes)
-// as its climbs end: no queue atomics, and a lane with none left exits.
-// `count` staged nodes bound the steps of one lane.
-__device__ __forceinline__ void climb_ready(Node* nodes, float4* box_lo, float4* box_hi,
-                                            const Staged& st, int count, lds<int>* s_list,
-                                            lds<unsigned>* s_cnt, int* error) {
-  constexpr unsigned kLanes = kClimbWaves * 64;
-  const unsigned nready = *st.nready;
-  unsigned r = threadIdx.x;
-  if (r >= nready) return;
-  int k = st.ready[r];
-  for (int step = 0;; ++step) {
-    if (step > count) {  // more combines than nodes: a malformed tree
-      *error = 1;
-      break;
-    }
-    int link = st.link[k];
-    int node = st.id[k];
-    const float4 c0 = st.com[2 * k], c1 = st.com[2 * k + 1];
-    const float3 l0 = st.lo[2 * k], l1 = st.lo[2 * k + 1];
-    const float3 h0 = st.hi[2 * k], h1 = st.hi[2 * k + 1];
-    asm volatile("" : "+v"(link), "+v"(node));  // issued with the slot reads, not after the combine
-    const NodeData cur = combine(NodeData{c0, make_float4(l0.x, l0.y, l0.z, 0.f),
-                                          make_float4(h0.x, h0.y, h0.z, 0.f)},
-                                 NodeData{c1, make_float4(l1.x, l1.y, l1.z, 0.f),
-                                          make_float4(h1.x, h1.y, h1.z, 0.f)});
-    store_node(nodes, box_lo, box_hi, node, cur);
-    const int pk = link & (kSideBit - 1);
-    bool next = false;
-    if (pk == kLinkRoot) {
-    } else if (pk == kLinkOut) {  // frontier entry of the next pass (none in the last pass)
-      if (s_list) seg_push(s_list, s_cnt, node, error);
-      else *error = 1;
-    } else if (arrive_staged(st, cur, pk, (link & kSideBit) ? 1 : 0, error)) {
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-      k = pk;
-      next = true;
-    }
-    if (!next) {  // this climb ended: the lane's next ready node, if any
-      r += kLanes;
-      if (r >= nready) break;
-      k = st.ready[r];
-    }
   }
 }
 
@@ -1021,13 +1022,15 @@ __device__ __forceinline__ void climb_ready(Node* nodes, float4* box_lo, float4*
 // instructions of a one-lane step (a lone wave issues one instruction per
 // four cycles: the step's instruction count is its latency).  Every decision
 // is uniform in the quad: lane 0 makes the arrival count after the whole wave
-// has completed its slot writes, and a quad permutation broadcasts it.
+// has completed its slot writes, and a quad permutation broadcasts it.  Quad
+// q takes ready nodes q, q + Q, q + 2Q, ... (Q climbing quads) as its climbs
+// end; `count` staged nodes bound the steps of one quad.
 template <int kPerm>  // DPP quad_perm control: 0x00 lane 0, 0x55 lane 1, 0xaa lane 2
 __device__ __forceinline__ float quad_bcast(float v) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), kPerm, 0xf, 0xf, false));
 }
 __device__ __forceinline__ void climb_ready4(Node* nodes, float4* box_lo, float4* box_hi,
-                                             const Staged& st, int count, lds<int>* s_list,
+                                             const Staged& st, int count, FrontEntry* out,
                                              lds<unsigned>* s_cnt, int* error) {
   constexpr unsigned kQuads = kClimbWaves * 16;
   const unsigned nready = *st.nready;
@@ -1039,9 +1042,9 @@ __device__ __forceinline__ void climb_ready4(Node* nodes, float4* box_lo, float4
   lds<float>* com = reinterpret_cast<lds<float>*>(st.com);
   lds<float>* lo = reinterpret_cast<lds<float>*>(st.lo);
   lds<float>* hi = reinterpret_cast<lds<float>*>(st.hi);
-  // Rare cases (non-finite operands, the root, a frontier parent, a malformed
-  // tree) are handled under wave-uniform branches, so the common step has no
-  // divergent control flow besides the end of a climb.
+  // Rare cases (non-finite operands, the root, a parent of a later pass, a
+  // malformed tree) are handled under wave-uniform branches, so the common
+  // step has no divergent control flow besides the end of a climb.
   for (int step = 0;; ++step) {
     if (step > count) {  // more combines than nodes: a malformed tree
       if (c == 0) *error = 1;
@@ -1072,7 +1075,7 @@ __device__ __forceinline__ void climb_ready4(Node* nodes, float4* box_lo, float4
     reinterpret_cast<float*>(box_hi + node)[c] = c == 3 ? 0.f : hmax;
     nodes[node].size2 = sz * sz;  // the same value from the four lanes
     const int pk = link & (kSideBit - 1);
-    const bool inner = pk < kLinkOut;  // the parent is staged in this pass
+    const bool inner = pk < kOutBase;  // the parent is staged in this pass
     int prev = 0;
     if (inner) {
       const int mine = 2 * pk + ((link & kSideBit) ? 1 : 0);
@@ -1086,12 +1089,24 @@ __device__ __forceinline__ void climb_ready4(Node* nodes, float4* box_lo, float4
       prev = __builtin_amdgcn_mov_dpp(prev, 0x00, 0xf, 0xf, false);  // lane 0's count
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
     }
-    if (__builtin_amdgcn_ballot_w64(!inner || prev > 1) && c == 0) {
+    if (__builtin_amdgcn_ballot_w64(!inner || prev > 1)) {
       if (inner && prev > 1) {  // a third arrival: a malformed tree
-        *error = 1;
-      } else if (!inner && pk == kLinkOut) {  // frontier entry of the next pass (not in the last)
-        if (s_list) seg_push(s_list, s_cnt, node, error);
-        else *error = 1;
+        if (c == 0) *error = 1;
+      } else if (!inner && pk != kLinkRoot) {  // the parent belongs to a later pass
+        int slot = 0;
+        if (c == 0)
+          slot = (int)__hip_atomic_fetch_add(s_cnt, 1u, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP);
+        slot = __builtin_amdgcn_mov_dpp(slot, 0x00, 0xf, 0xf, false);
+        if (!out || slot >= kSeg) {
+          if (c == 0) *error = 1;
+        } else {  // the entry's twelve words, three per lane
+          const int elink = (pk - kOutBase) | (link & kSideBit);
+          float* e = reinterpret_cast<float*>(out + slot);
+          e[c] = q;
+          e[4 + c] = c == 3 ? __int_as_float(elink) : lmin;
+          e[8 + c] = c == 3 ? __int_as_float(node) : hmax;
+        }
       }
     }
     if (inner && prev == 1) {
@@ -1105,20 +1120,22 @@ __device__ __forceinline__ void climb_ready4(Node* nodes, float4* box_lo, float4
 }
 
 // Pass-3 node list, filled by pass 2 (one reservation per workgroup) and
-// read and reset by pass 3.
+// read and reset by pass 3: (id, link) per node.
 struct TopList {
   unsigned* count;
-  int* list;
+  int2* list;
 };
 
-// Pass 2: workgroup s owns the superblock of leaves [s*S, s*S + S): it climbs
-// from the frontier segments of its pass-1 workgroups.  Its own nodes are the
-// internal ids of the superblock whose visit word says pass 2 (an internal
-// node's id lies in its leaf range); it finds them with one coalesced read of
-// the 32,768 words, stages them (at most kMidCap; usually about 350) with a
-// 16-bit id -> index table, and lists the pass-3 ids it sees for pass 3.  A
+// Pass 2: workgroup s owns the superblock of leaves [s*S, s*S + S) and its
+// kPer = S / L pass-1 workgroups.  Thread t serves pass-1 workgroup t / 32,
+// slots t % 32 (then t % 32 + 32, ... in the rare longer segments): it loads
+// that workgroup's two counts, its node record and its frontier entry at the
+// start, so the pass waits for one global round trip before it stages.  The
+// records of pass 2 are staged (at most kMidCap; usually about 350) with a
+// 16-bit id -> index table, those of pass 3 are listed for pass 3.  A
 // superblock with more pass-2 nodes climbs with 16-bit arrival words in LDS
-// and exchange slots instead (climb_exchange).
+// and exchange slots instead (climb_exchange from the entries' ids, reading
+// their data from global memory).
 constexpr int kMidCap = 640;
 constexpr int kMidSlots = 1024;
 constexpr size_t kSlotBytes = 16 + 12 + 12;  // COM float4, lo float3, hi float3
@@ -1126,19 +1143,30 @@ constexpr size_t kMidStagedBytes = 2 * kMidCap * kSlotBytes + 4 * kMidCap * 4 + 
 constexpr size_t kMidUnstagedBytes = kMidSlots * kSlotBytes + kComSuper / 2 * 4;
 constexpr size_t kMidBytes = kMidStagedBytes > kMidUnstagedBytes ? kMidStagedBytes
                                                                  : kMidUnstagedBytes;
-constexpr int kMidTops = 256;  // pass-3 ids a superblock collects before listing them
+constexpr int kMidTops = 256;  // pass-3 nodes a superblock collects before listing them
 __global__ __launch_bounds__(kComThreads) void com_mid_kernel(
     const float4* __restrict__ pos_sorted, int n, Node* nodes, const int4* __restrict__ up,
-    const unsigned* __restrict__ visit, float4* box_lo, float4* box_hi,
-    const int* __restrict__ seg1, const unsigned* __restrict__ cnt1, int nblocks1, int* seg2,
-    unsigned* cnt2, TopList top, int* error) {
+    float4* box_lo, float4* box_hi, const FrontEntry* __restrict__ seg1,
+    const unsigned* __restrict__ cnt1, const int2* __restrict__ rec1,
+    const unsigned* __restrict__ rcnt1, int nblocks1, FrontEntry* seg2, unsigned* cnt2,
+    TopList top, int* error) {
   __shared__ __align__(16) unsigned char s_raw[kMidBytes];
-  __shared__ int s_front[kSeg];
-  __shared__ int s_tops[kMidTops];
-  __shared__ unsigned s_nfront, s_used, s_nmid, s_ntop, s_topbase, s_nready, s_head;
-  const int nm1 = n - 1;
+  __shared__ int2 s_tops[kMidTops];
+  __shared__ unsigned s_nfront, s_used, s_nmid, s_ntop, s_topbase, s_nready;
   const int lo = blockIdx.x * kComSuper;
   const int hi = min(n - 1, lo + kComSuper - 1);
+  constexpr int kPer = kComSuper / kComLeaves;  // pass-1 workgroups per superblock
+  static_assert(kPer * 32 == kComThreads, "32 threads per pass-1 workgroup");
+  // this thread's pass-1 workgroup and first slot; every load is issued
+  // before the first is used (clamped, so none waits for a branch)
+  const int b1 = blockIdx.x * kPer + (int)(threadIdx.x >> 5);
+  const int slot0 = threadIdx.x & 31;
+  const int bq = min(b1, nblocks1 - 1);
+  const unsigned nrec_raw = rcnt1[bq], nent_raw = cnt1[bq];
+  const int2 rec0 = rec1[(size_t)bq * kSeg + slot0];
+  const FrontEntry ent0 = seg1[(size_t)bq * kSeg + slot0];
+  const unsigned nrec = b1 < nblocks1 ? min(nrec_raw, (unsigned)kSeg) : 0u;
+  const unsigned nent = b1 < nblocks1 ? min(nent_raw, (unsigned)kSeg) : 0u;
   // staged layout
   lds<unsigned char>* raw = (lds<unsigned char>*)s_raw;
   Staged st;
@@ -1150,7 +1178,6 @@ __global__ __launch_bounds__(kComThreads) void com_mid_kernel(
   st.arrived = reinterpret_cast<lds<unsigned>*>(st.link + kMidCap);
   st.ready = reinterpret_cast<lds<int>*>(st.arrived + kMidCap);
   st.nready = (lds<unsigned>*)&s_nready;
-  st.head = (lds<unsigned>*)&s_head;
   lds<unsigned short>* s_index = reinterpret_cast<lds<unsigned short>*>(st.ready + kMidCap);
   // unstaged layout
   const ComSlots slots{reinterpret_cast<lds<float4>*>(raw),
@@ -1167,56 +1194,31 @@ __global__ __launch_bounds__(kComThreads) void com_mid_kernel(
     s_nmid = 0;
     s_ntop = 0;
     s_nready = 0;
-    s_head = 0;
   }
   __syncthreads();
-  // this superblock's pass-2 nodes (into st.id) and pass-3 ids; every load
-  // is issued before the first is used, and a block-wide scan of the
-  // per-thread counts places them (no atomic per found node)
-  constexpr int kScan = kComSuper / kComThreads;
-  static_assert(kScan <= 32, "one 32-bit mask per thread");
-  const int last = min(hi, nm1 - 1);
-  unsigned word[kScan];
-#pragma unroll
-  for (int r = 0; r < kScan; ++r)  // unconditional (clamped) loads, so none waits for a branch
-    word[r] = visit[min(lo + r * kComThreads + (int)threadIdx.x, last)];
-  unsigned mid_mask = 0, top_mask = 0;
-#pragma unroll
-  for (int r = 0; r < kScan; ++r) {
-    const int i = lo + r * kComThreads + (int)threadIdx.x;
-    const unsigned pass = i <= last ? word[r] >> 30 : 0u;
-    mid_mask |= (pass == 1 ? 1u : 0u) << r;
-    top_mask |= (pass == 2 ? 1u : 0u) << r;
-  }
-  // both counts in one word: each is at most kComSuper < 2^16
-  static_assert(kComSuper < (1 << 16), "16-bit counts");
-  const unsigned packed = (unsigned)__popc(mid_mask) | ((unsigned)__popc(top_mask) << 16);
-  const unsigned incl = wave_incl_scan(packed);
-  __shared__ unsigned s_wsum[kComThreads / 64];
-  if ((threadIdx.x & 63) == 63) s_wsum[threadIdx.x >> 6] = incl;
-  __syncthreads();
-  unsigned base = incl - packed;
-  for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) base += s_wsum[w];
-  if (threadIdx.x == kComThreads - 1) {
-    s_nmid = (base + packed) & 0xffffu;
-    s_ntop = (base + packed) >> 16;
-  }
-  unsigned m1 = base & 0xffffu, m2 = base >> 16;
-#pragma unroll
-  for (int r = 0; r < kScan; ++r) {
-    const int i = lo + r * kComThreads + (int)threadIdx.x;
-    if ((mid_mask >> r) & 1u) {
-      if (m1 < (unsigned)kMidCap) st.id[m1] = i;
-      ++m1;
-    }
-    if ((top_mask >> r) & 1u) {
-      if (m2 < (unsigned)kMidTops) {
-        s_tops[m2] = i;
-      } else {  // many pass-3 ids in one superblock: list this one alone
-        const unsigned g = atomic_add_relaxed(top.count, 1u);
-        if (g < (unsigned)kTopList) top.list[g] = i;
+  for (unsigned j = slot0; j < nrec; j += 32) {
+    const int2 r = j == (unsigned)slot0 ? rec0 : rec1[(size_t)bq * kSeg + j];
+    const unsigned pass = (unsigned)r.x >> 30;
+    const int id = r.x & kIdMask;
+    if (pass == 1 && id >= lo && id <= hi) {
+      const unsigned k = __hip_atomic_fetch_add(&s_nmid, 1u, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (k < (unsigned)kMidCap) {
+        st.id[k] = id;
+        st.link[k] = r.y;
+        s_index[id - lo] = (unsigned short)(k + 1);
       }
-      ++m2;
+    } else if (pass == 2) {
+      const unsigned k = __hip_atomic_fetch_add(&s_ntop, 1u, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (k < (unsigned)kMidTops) {
+        s_tops[k] = make_int2(id, r.y);
+      } else {  // many pass-3 nodes in one superblock: list this one alone
+        const unsigned g = atomic_add_relaxed(top.count, 1u);
+        if (g < (unsigned)kTopList) top.list[g] = make_int2(id, r.y);
+      }
+    } else {  // a record of pass 1, or of a node outside the superblock
+      *error = 1;
     }
   }
   __syncthreads();
@@ -1224,40 +1226,18 @@ __global__ __launch_bounds__(kComThreads) void com_mid_kernel(
   const bool staged = nmid <= (unsigned)kMidCap;
   const unsigned ntop = min(s_ntop, (unsigned)kMidTops);
   if (threadIdx.x == 0 && ntop > 0) s_topbase = atomic_add_relaxed(top.count, ntop);
-  constexpr int kPer = kComSuper / kComLeaves;  // pass-1 workgroups per superblock
-  const int b0 = blockIdx.x * kPer;
-  // compact the segments' entries so that entry j goes to thread j: one wave
-  // loads the kPer counts together and scans them
-  static_assert(kPer <= 64, "the segment scan is one wave wide");
-  __shared__ unsigned s_off[kPer + 1];
-  if (threadIdx.x < 64) {
-    const int b = threadIdx.x;
-    const unsigned c = (b < kPer && b0 + b < nblocks1) ? cnt1[b0 + b] : 0u;
-    const unsigned incl = wave_incl_scan(c);
-    if (b < kPer) s_off[b + 1] = incl;
-    if (b == 0) s_off[0] = 0;
-  }
-  __syncthreads();
-  if (ntop > 0 && threadIdx.x < ntop && s_topbase + threadIdx.x < (unsigned)kTopList)
-    top.list[s_topbase + threadIdx.x] = s_tops[threadIdx.x];
   auto index = [&](int id) {
     const int rel = id - lo;
     return (rel >= 0 && rel < kComSuper) ? (int)s_index[rel] - 1 : -1;
   };
   if (staged) {
-    for (int k = threadIdx.x; k < (int)nmid; k += blockDim.x) {
-      const int id = st.id[k];
-      const int4 u = up[id];
-      st.link[k] = u.x;                  // parent id until link_staged
-      st.arrived[k] = id < u.y ? 0u : 1u;  // side until link_staged
-      s_index[id - lo] = (unsigned short)(k + 1);
-    }
-    __syncthreads();
     link_staged(st, (int)nmid, index);
   } else {
     for (int k = threadIdx.x; k < kComSuper / 2; k += blockDim.x) s_arrive[k] = 0;
   }
   __syncthreads();
+  if (ntop > 0 && threadIdx.x < ntop && s_topbase + threadIdx.x < (unsigned)kTopList)
+    top.list[s_topbase + threadIdx.x] = s_tops[threadIdx.x];
   auto arrive = [&](int node, unsigned tag) {
     const int k = node - lo;
     const unsigned shift = 16u * (unsigned)(k & 1);
@@ -1265,35 +1245,31 @@ __global__ __launch_bounds__(kComThreads) void com_mid_kernel(
                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     return (old >> shift) & 0xffffu;
   };
-  lds<int>* front = (lds<int>*)s_front;
+  FrontEntry* out = seg2 + (size_t)blockIdx.x * kSeg;
   lds<unsigned>* nfront = (lds<unsigned>*)&s_nfront;
-  for (unsigned j = threadIdx.x; j < s_off[kPer]; j += blockDim.x) {
-    int b = 0;
-    for (int step = kPer / 2; step > 0; step >>= 1)  // last b with s_off[b] <= j
-      if (s_off[b + step] <= j) b += step;
-    const int child = seg1[(b0 + b) * kSeg + (j - s_off[b])];
+  for (unsigned j = slot0; j < nent; j += 32) {
+    const FrontEntry e = j == (unsigned)slot0 ? ent0 : seg1[(size_t)bq * kSeg + j];
     if (staged)
-      arrive_from(pos_sorted, n, nodes, up, box_lo, box_hi, child, st, index, front, nfront,
-                  error);
+      arrive_entry(e, st, index, out, nfront, error);
     else
-      climb_exchange(pos_sorted, n, nodes, up, box_lo, box_hi, child, lo, hi, front, nfront,
-                     slots, arrive, error);
+      climb_exchange(pos_sorted, n, nodes, up, box_lo, box_hi, __float_as_int(e.hi.w), lo, hi,
+                     out, nfront, slots, arrive, error);
   }
   if (staged) {  // uniform: nmid is one value for the block
     __syncthreads();
     if (threadIdx.x < kClimbWaves * 64)
-      climb_ready4(nodes, box_lo, box_hi, st, (int)nmid, front, nfront, error);
+      climb_ready4(nodes, box_lo, box_hi, st, (int)nmid, out, nfront, error);
   }
   __syncthreads();
-  const unsigned cnt = min(s_nfront, (unsigned)kSeg);
-  if (threadIdx.x < cnt) seg2[blockIdx.x * kSeg + threadIdx.x] = s_front[threadIdx.x];
-  if (threadIdx.x == 0) cnt2[blockIdx.x] = cnt;
+  if (threadIdx.x == 0) cnt2[blockIdx.x] = min(s_nfront, (unsigned)kSeg);
 }
 
 // Pass 3: one workgroup climbs from every superblock's frontier segment
 // through the nodes that span superblocks (about 520 at 1M leaves), staged
-// from the list pass 2 made, with an LDS hash from id to index.  With more
-// than kTopCap listed nodes it climbs with arrival tags in the low bits of the
+// from the list pass 2 made, with an LDS hash from id to index.  Thread t
+// serves segment t / 32 (+ 32, 64, ...), slots t % 32 (+ 32, ...), and its
+// first entry and list record are loaded at the start.  With more than
+// kTopCap listed nodes it climbs with arrival tags in the low bits of the
 // per-node `visit` words in global memory (written by the Karras kernel with
 // those bits clear; one workgroup, so workgroup scope holds) and exchange
 // slots instead.  Earlier passes are visible across the kernel boundaries.
@@ -1308,16 +1284,22 @@ __device__ __forceinline__ unsigned top_hash(int id) {
   return ((unsigned)id * 2654435761u) >> (32 - 11);
 }
 static_assert(kTopHash == 1 << 11, "top_hash yields 11 bits");
+static_assert(kTopCap <= kComThreads, "one listed node per thread in the first load");
 
 __global__ __launch_bounds__(kComThreads) void com_top_kernel(
     const float4* __restrict__ pos_sorted, int n, Node* nodes, const int4* __restrict__ up,
-    unsigned* visit, float4* box_lo, float4* box_hi, const int* __restrict__ seg2,
+    unsigned* visit, float4* box_lo, float4* box_hi, const FrontEntry* __restrict__ seg2,
     const unsigned* __restrict__ cnt2, int nblocks2, TopList top, int* error) {
   __shared__ __align__(16) unsigned char s_raw[kTopBytes];
-  __shared__ unsigned s_used, s_nready, s_head;
-  const int nm1 = n - 1;
+  __shared__ unsigned s_used, s_nready;
   const unsigned listed = *top.count;
   const bool staged = listed <= (unsigned)kTopCap;
+  // first loads: this thread's listed node and its first entry
+  const int2 rec0 = top.list[threadIdx.x];
+  const int b0 = threadIdx.x >> 5, slot0 = threadIdx.x & 31;
+  const int bq = min(b0, nblocks2 - 1);
+  const unsigned nent0_raw = cnt2[bq];
+  const FrontEntry ent0 = seg2[(size_t)bq * kSeg + slot0];
   lds<unsigned char>* raw = (lds<unsigned char>*)s_raw;
   Staged st;
   st.com = reinterpret_cast<lds<float4>*>(raw);
@@ -1328,7 +1310,6 @@ __global__ __launch_bounds__(kComThreads) void com_top_kernel(
   st.arrived = reinterpret_cast<lds<unsigned>*>(st.link + kTopCap);
   st.ready = reinterpret_cast<lds<int>*>(st.arrived + kTopCap);
   st.nready = (lds<unsigned>*)&s_nready;
-  st.head = (lds<unsigned>*)&s_head;
   lds<int>* s_key = reinterpret_cast<lds<int>*>(st.ready + kTopCap);
   lds<int>* s_val = s_key + kTopHash;
   const ComSlots slots{reinterpret_cast<lds<float4>*>(raw),
@@ -1338,45 +1319,20 @@ __global__ __launch_bounds__(kComThreads) void com_top_kernel(
   if (threadIdx.x == 0) {
     s_used = 0;
     s_nready = 0;
-    s_head = 0;
   }
   if (staged)
     for (int k = threadIdx.x; k < kTopHash; k += blockDim.x) s_key[k] = -1;
-  // compact the entries of all superblock segments so entry j goes to thread
-  // j: each thread loads kTopPer consecutive counts, then a block-wide scan
-  constexpr int kTopPer = kTopSegs / kComThreads;
-  constexpr int kWaves = kComThreads / 64;
-  __shared__ unsigned s_off[kTopSegs + 1];
-  __shared__ unsigned s_wave[kWaves];
-  unsigned c[kTopPer];
-  unsigned sum = 0;
-  for (int k = 0; k < kTopPer; ++k) {
-    const int b = threadIdx.x * kTopPer + k;
-    c[k] = b < nblocks2 ? cnt2[b] : 0u;
-    sum += c[k];
-  }
-  const unsigned incl = wave_incl_scan(sum);
-  const int wave = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 63) s_wave[wave] = incl;
   __syncthreads();
-  unsigned base = incl - sum;
-  for (int w = 0; w < wave; ++w) base += s_wave[w];
-  for (int k = 0; k < kTopPer; ++k) {
-    s_off[threadIdx.x * kTopPer + k] = base;
-    base += c[k];
-  }
-  if (threadIdx.x == kComThreads - 1) s_off[kTopSegs] = base;
-  if (staged) {
-    for (int k = threadIdx.x; k < (int)listed; k += blockDim.x) {
-      const int id = top.list[k];
-      if (id < 0 || id >= nm1) {  // only on a stale or corrupt list
-        *error = 1;
-        continue;
-      }
-      const int4 u = up[id];
+  if (staged && threadIdx.x < listed) {
+    const int k = threadIdx.x;
+    const int id = rec0.x;
+    if (id < 0 || id >= n - 1) {  // only on a stale or corrupt list
+      *error = 1;
+      st.id[k] = 0;  // keeps every later access in range
+      st.link[k] = -1;
+    } else {
       st.id[k] = id;
-      st.link[k] = u.x;                  // parent id until link_staged
-      st.arrived[k] = id < u.y ? 0u : 1u;  // side until link_staged
+      st.link[k] = rec0.y;
       unsigned h = top_hash(id);
       bool placed = false;
       for (int probe = 0; probe < kTopHash && !placed; ++probe) {
@@ -1411,22 +1367,19 @@ __global__ __launch_bounds__(kComThreads) void com_top_kernel(
     return __hip_atomic_fetch_or(&visit[node], tag, __ATOMIC_RELAXED,
                                  __HIP_MEMORY_SCOPE_WORKGROUP) & 0xffffu;
   };
-  const unsigned total_entries = s_off[nblocks2];
-  int top_step = 1;  // largest power of two <= nblocks2 (usually 32 segments, not kTopSegs)
-  while (top_step * 2 <= nblocks2) top_step *= 2;
-  for (unsigned j = threadIdx.x; j < total_entries; j += blockDim.x) {
-    int b = 0;
-    for (int step = top_step; step > 0; step >>= 1)  // last b with s_off[b] <= j
-      if (b + step <= nblocks2 && s_off[b + step] <= j) b += step;
-    const int child = seg2[b * kSeg + (j - s_off[b])];
-    // the whole tree is in range here: no climb stops at a frontier (a
-    // staged climb whose parent is not listed has a corrupt list)
-    if (staged)
-      arrive_from(pos_sorted, n, nodes, up, box_lo, box_hi, child, st, index, nullptr, nullptr,
-                  error);
-    else
-      climb_exchange(pos_sorted, n, nodes, up, box_lo, box_hi, child, 0, n - 1, nullptr,
-                     nullptr, slots, arrive, error);
+  // the whole tree is in range here: an entry or climb whose parent is not
+  // listed means a corrupt list
+  for (int b = b0; b < nblocks2; b += kComThreads / 32) {
+    const unsigned nent = min(b == b0 ? nent0_raw : cnt2[b], (unsigned)kSeg);
+    for (unsigned j = slot0; j < nent; j += 32) {
+      const FrontEntry e =
+          (b == b0 && j == (unsigned)slot0) ? ent0 : seg2[(size_t)b * kSeg + j];
+      if (staged)
+        arrive_entry(e, st, index, nullptr, nullptr, error);
+      else
+        climb_exchange(pos_sorted, n, nodes, up, box_lo, box_hi, __float_as_int(e.hi.w), 0,
+                       n - 1, nullptr, nullptr, slots, arrive, error);
+    }
   }
   if (staged) {  // uniform: listed is one value for the block
     __syncthreads();
@@ -1707,8 +1660,10 @@ int com_g2(int n) { return (n + kComSuper - 1) / kComSuper; }
 }  // namespace
 
 int com_scratch_ints(int n) {
-  const int g1 = com_g1(n), g2 = com_g2(n);
-  return kComCounterInts + kTopList + (g1 + g2) * (kSeg + 1);
+  const size_t g1 = (size_t)com_g1(n), g2 = (size_t)com_g2(n);
+  const size_t ints = kComCounterInts + 2 * (size_t)kTopList +
+                      (g1 + g2) * kSeg * (sizeof(FrontEntry) / 4) + g1 * kSeg * 2 + 2 * g1 + g2;
+  return (int)ints;
 }
 
 void launch_com(hipStream_t s, const float4* pos_sorted, int n, Node* nodes, const int4* up,
@@ -1716,17 +1671,21 @@ void launch_com(hipStream_t s, const float4* pos_sorted, int n, Node* nodes, con
   if (n < 2) return;
   const int g1 = com_g1(n), g2 = com_g2(n);
   if (g2 > kTopSegs) throw Error(kErrInvalid, "particle count too large for the COM top pass");
-  const TopList top{reinterpret_cast<unsigned*>(scratch), scratch + kComCounterInts};
-  int* seg1 = top.list + kTopList;
-  int* seg2 = seg1 + g1 * kSeg;
-  unsigned* cnt1 = reinterpret_cast<unsigned*>(seg2 + g2 * kSeg);
-  unsigned* cnt2 = cnt1 + g1;
-  com_local_kernel<<<g1, kComThreads, 0, s>>>(pos_sorted, n, nodes, up, box_lo, box_hi, seg1,
-                                              cnt1, error);
+  static_assert(kComCounterInts % 4 == 0 && sizeof(FrontEntry) == 48, "16-byte aligned carve");
+  const TopList top{reinterpret_cast<unsigned*>(scratch),
+                    reinterpret_cast<int2*>(scratch + kComCounterInts)};
+  FrontEntry* seg1 = reinterpret_cast<FrontEntry*>(top.list + kTopList);
+  FrontEntry* seg2 = seg1 + (size_t)g1 * kSeg;
+  int2* rec1 = reinterpret_cast<int2*>(seg2 + (size_t)g2 * kSeg);
+  unsigned* cnt1 = reinterpret_cast<unsigned*>(rec1 + (size_t)g1 * kSeg);
+  unsigned* rcnt1 = cnt1 + g1;
+  unsigned* cnt2 = rcnt1 + g1;
+  com_local_kernel<<<g1, kComThreads, 0, s>>>(pos_sorted, n, nodes, up, visit, box_lo, box_hi,
+                                              seg1, cnt1, rec1, rcnt1, error);
   NB_LAUNCH_CHECK();
   if (g1 == 1) return;
-  com_mid_kernel<<<g2, kComThreads, 0, s>>>(pos_sorted, n, nodes, up, visit, box_lo, box_hi, seg1,
-                                            cnt1, g1, seg2, cnt2, top, error);
+  com_mid_kernel<<<g2, kComThreads, 0, s>>>(pos_sorted, n, nodes, up, box_lo, box_hi, seg1, cnt1,
+                                            rec1, rcnt1, g1, seg2, cnt2, top, error);
   NB_LAUNCH_CHECK();
   if (g2 == 1) return;
   com_top_kernel<<<1, kComThreads, 0, s>>>(pos_sorted, n, nodes, up, visit, box_lo, box_hi, seg2,

@@ -198,13 +198,16 @@ constexpr int kComLeaves = 1024;
 constexpr int kComSuper = 32768;
 constexpr int kSeg = 256;         // frontier entries per workgroup segment
 constexpr int kTopList = 4096;    // listed pass-3 nodes
-// Scratch of the COM passes (ints, com_scratch_ints(n) of them):
-//   topcnt[1]                  listed pass-3 nodes (zero between builds)
-//   toplist[kTopList]
-//   seg1[g1 * kSeg], seg2[g2 * kSeg], cnt1[g1], cnt2[g2]   frontier segments
+// Scratch of the COM passes (ints, com_scratch_ints(n) of them; 16-byte
+// aligned parts):
+//   topcnt[kComCounterInts]    listed pass-3 nodes in word 0 (zero between builds)
+//   toplist int2[kTopList]     (id, link) of the listed pass-3 nodes
+//   seg1[g1 * kSeg], seg2[g2 * kSeg]   frontier entries (48 bytes: data, link, id)
+//   rec1 int2[g1 * kSeg]       pass 1's records of later-pass nodes
+//   cnt1[g1], rcnt1[g1], cnt2[g2]      segment counts
 // with g1 = ceil(n / kComLeaves), g2 = ceil(n / kComSuper).
 int com_scratch_ints(int n);
-constexpr int kComCounterInts = 1;  // words at the start a new buffer must have zeroed
+constexpr int kComCounterInts = 4;  // words at the start a new buffer must have zeroed
 // The `visit` word of internal node i of leaf range [first, last]: its COM
 // pass (0, 1 or 2) in bits 30-31, arrival bits clear.
 __device__ __forceinline__ unsigned com_visit_word(int first, int last) {

@@ -101,7 +101,7 @@ def test_tree_pipeline_bit_exact(case, bits):
 @pytest.mark.parametrize("n", [200003, 1200000])
 def test_com_passes_bit_exact(n):
     """COM is bit-exact to the oracle through all three passes: 1024-leaf
-    blocks, 65,536-leaf superblocks and the top (19 superblocks at 1.2M)."""
+    blocks, 32,768-leaf superblocks and the top (37 superblocks at 1.2M)."""
     pos = ic.two_disk_encounter(n, seed=9, eps=0.02)[0]
     print(f"com passes n={n}", flush=True)
     with engine.Simulation(pos, eps=0.02, theta=0.5, force="bh", traverse="warp",
