@@ -130,7 +130,7 @@ BhScratch carve_bh_scratch(void* base, int n, size_t* bytes) {
   s.bbox_partial = reinterpret_cast<float4*>(take(2 * (size_t)kBboxBlocks * sizeof(float4)));
   s.max_tiles = sort_max_tiles((int)c);
   s.lookback = reinterpret_cast<unsigned long long*>(
-      take((size_t)kHistWords * s.max_tiles * sizeof(unsigned long long)));
+      take(lookback_words(s.max_tiles) * sizeof(unsigned long long)));
   s.keys_a = reinterpret_cast<uint64_t*>(take(c * sizeof(uint64_t)));
   s.keys_b = reinterpret_cast<uint64_t*>(take(c * sizeof(uint64_t)));
   s.vals_a = reinterpret_cast<unsigned*>(take(c * sizeof(unsigned)));
@@ -162,12 +162,13 @@ void enqueue_bh_force(hipStream_t stream, const BhScratch& b, float4* pos, float
                      sort_input(p.key_bits, b.keys_a, b.keys_b), b.state);
   mark(1);
   launch_sort(stream, p.key_bits, b.keys_a, b.keys_b, b.vals_a, b.vals_b, n, b.state, b.lookback,
-              b.max_tiles, true, pos, b.pos_sorted);
+              b.max_tiles, true, nullptr, nullptr);
   mark(2);
   launch_karras(stream, p.key_bits, b.keys_a, n, b.nodes, b.up, b.visit, b.state);
   mark(3);
+  // the gather of the sorted positions runs in the first COM pass
   launch_com(stream, b.pos_sorted, n, b.nodes, b.up, b.visit, b.box_lo, b.box_hi, b.frontier,
-             &b.state->error);
+             &b.state->error, b.vals_a, pos);
   mark(4);
   WalkArgs w{};
   w.pos_sorted = b.pos_sorted;
@@ -806,7 +807,7 @@ class Engine {
     quant.alloc(1, true);
     state.alloc(1, true);
     max_tiles = sort_max_tiles(n);
-    lookback.alloc((size_t)kHistWords * max_tiles, true);
+    lookback.alloc(lookback_words(max_tiles), true);
     // the sort state is reset by each step's Karras kernel for the next step;
     // start from a reset state on this stream
     launch_sort_init(stream, state.p);
@@ -1496,7 +1497,7 @@ int nb_sort_pairs(const uint64_t* keys, int n, int key_bits, uint64_t* keys_sort
     va.alloc(n);
     vb.alloc(n);
     ss.alloc(1, true);
-    lb.alloc((size_t)nb::kHistWords * tiles, true);
+    lb.alloc(nb::lookback_words(tiles), true);
     void* in = nb::sort_input(key_bits, ka.p, kb.p);
     if (key_bits == 30) {
       std::vector<uint32_t> k(n);

@@ -11,16 +11,18 @@
 // Sort (replaces CUB SortPairs, forces_bh.cu:490-497): LSD radix sort, 8-bit
 // digits, one kernel per pass.  Each 256-thread workgroup takes a ticket-
 // ordered tile of keys, ranks its keys stably in LDS (per-wave ballot match
-// of equal digits), publishes per-digit tile counts and resolves its global
-// offsets by decoupled look-back over earlier tiles only (earlier tickets
-// are already resident, so the wait always ends).  The tile is then staged
-// in LDS in sorted order and written out in that order, so consecutive
-// threads write consecutive positions of each digit's run (1M 30-bit keys:
-// 95.5 us for the four passes against 103.8 scattering from registers).
-// Stable: ties keep input order, so equal keys end in ascending original
-// index.  A 1M-key pass is one tile per CU, bound by latency: ~5 us to load
-// and rank, ~9 us until inclusive prefixes have propagated through the 256
-// tiles, ~2 us to write, ~3 us between launches.
+// of equal digits), publishes its per-digit tile counts at once and resolves
+// its global offsets from earlier tiles only (earlier tickets are already
+// resident, so the wait always ends): up to 1024 tiles by a two-level sum
+// (the earlier tiles of its group of ~sqrt(tiles), then the earlier groups'
+// totals), beyond that by decoupled look-back.  The tile is then staged in
+// LDS in sorted order and written out in that order, so consecutive threads
+// write consecutive positions of each digit's run.  Stable: ties keep input
+// order, so equal keys end in ascending original index.  A 1M-key pass is one
+// tile per CU, bound by latency: ~3 us until the last workgroup starts, ~5 us
+// to load and rank, ~1.4 us to stage, ~4 us until the prefix is summed, ~2 us
+// to write, ~3 us between launches (1M 30-bit keys: 76 us for the four passes;
+// 95.5 us with the look-back chain, which took ~8.7 us per pass).
 //
 // Karras build (forces_bh.cu:234-277): one thread per internal node.
 // COM (forces_bh.cu:281-331): leaves climb towards the root and the second
@@ -174,6 +176,54 @@ struct alignas(4 * D) Words {
   unsigned v[D];
 };
 
+// Sum of the `cnt` (< 64) published aggregates at p, p + RADIX, ... (D
+// consecutive digits each), polling those not yet published.  A round issues
+// its loads 16 at a time before it reads any of them.  A word that stays
+// stale for kLookbackSpins rounds (about a second) means the sort state was
+// not reset: the tile reports an error and goes on, so no tile waits forever.
+template <int D, int RADIX>
+__device__ __forceinline__ void sum_published(unsigned long long* p, int cnt, unsigned (&sum)[D],
+                                              unsigned epoch, SortState* ss) {
+  unsigned long long pending[D];
+#pragma unroll
+  for (int e = 0; e < D; ++e) pending[e] = cnt >= 64 ? ~0ull : (1ull << cnt) - 1ull;
+  for (unsigned spins = 0;; ++spins) {
+    for (int j0 = 0; j0 < cnt; j0 += 16) {
+      unsigned long long v[16][D];
+#pragma unroll
+      for (int i = 0; i < 16; ++i)
+#pragma unroll
+        for (int e = 0; e < D; ++e)
+          v[i][e] = (j0 + i < cnt && ((pending[e] >> (j0 + i)) & 1ull))
+                        ? load_agent(p + (size_t)(j0 + i) * RADIX + e)
+                        : 0ull;
+#pragma unroll
+      for (int i = 0; i < 16; ++i)
+#pragma unroll
+        for (int e = 0; e < D; ++e) {
+          const unsigned long long x = v[i][e];
+          if ((unsigned)(x >> 34) == epoch && (x & (3ull << 32)) != 0) {
+            sum[e] += (unsigned)x;
+            pending[e] &= ~(1ull << (j0 + i));
+          }
+        }
+    }
+    unsigned long long any = 0;
+#pragma unroll
+    for (int e = 0; e < D; ++e) any |= pending[e];
+    if (!any) break;
+    if (spins >= kLookbackSpins) {
+      ss->error = 1;
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+// Tiles up to which the onesweep pass takes the two-level prefix below
+// instead of the decoupled look-back chain (1024 tiles: 4M keys).
+constexpr int kGroupPrefixTiles = 1024;
+
 template <typename K, int BITS, bool FIRST, bool GATHER, int ITEMS>
 __global__ __launch_bounds__(kBlock) void onesweep_kernel(
     const K* __restrict__ kin, const unsigned* __restrict__ vin, K* __restrict__ kout,
@@ -186,8 +236,9 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
   constexpr int TILE = kBlock * ITEMS;
   constexpr int D = RADIX / kBlock;  // consecutive digits owned by each thread
   static_assert(D * kBlock == RADIX && (D == 1 || D == 2 || D == 4), "digits per thread");
-  // look-back window: earlier tiles read at once per digit (1M 30-bit keys:
-  // 95.5 us for the sort reading 4, 99.0 reading 1, 97.9 reading 8)
+  // look-back window of the chain: earlier tiles read at once per digit
+  // (1M 30-bit keys: 95.5 us for the sort reading 4, 99.0 reading 1, 97.9
+  // reading 8)
   constexpr int LB = D == 1 ? 4 : 1;
   using W = Words<D>;
   __shared__ __align__(16) unsigned s_cnt[WAVES][RADIX];
@@ -255,6 +306,18 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
     }
     *row = o;
   }
+  // Publish this tile's digit totals at once, so that the scans and the
+  // staging below overlap the other tiles' wait for them.  Per pass the
+  // look-back area holds max_tiles rows of tile totals and max_tiles rows of
+  // group totals, all words epoch-tagged (never cleared).
+  unsigned long long* lbrow = lookback + (size_t)pass * 2 * max_tiles * RADIX + tid * D;
+  const unsigned long long tag = (unsigned long long)epoch << 34;
+  const int ntiles = (int)gridDim.x;
+  const bool grouped = ntiles <= kGroupPrefixTiles;
+#pragma unroll
+  for (int e = 0; e < D; ++e)
+    store_agent(lbrow + (size_t)tile * RADIX + e,
+                tag | (grouped || tile != 0 ? kFlagAgg : kFlagInc) | total.v[e]);
   // exclusive scans over the digits: where each digit starts in the tile
   // (tile totals) and in the output (the histogram)
   unsigned tsum = 0, hsum = 0;
@@ -297,22 +360,40 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
     s_val[at] = val[j];
   }
 
-  // decoupled look-back over earlier tiles for this thread's digits, LB
-  // tiles at a time: the window's words are loaded together, then summed
-  // from the nearest tile back to the first inclusive count or to the
-  // first word not yet published (read again on the next round)
-  unsigned long long* row = lookback + (size_t)pass * max_tiles * RADIX + tid * D;
-  const unsigned long long tag = (unsigned long long)epoch << 34;
+  // Exclusive prefix of this tile's digit counts over the earlier tiles.
+  unsigned long long* row = lbrow;
   unsigned excl[D];
 #pragma unroll
   for (int e = 0; e < D; ++e) excl[e] = 0;
-  if (tile == 0) {
+  if (grouped) {
+    // Two levels: tiles in groups of G = ceil(sqrt(tiles)).  A tile sums
+    // the aggregates of its group's earlier tiles; the group's last tile
+    // publishes the group total; every tile sums the earlier groups'
+    // totals.  Each level's words are loaded together, so the prefix takes a
+    // few memory round trips where the chain below waits on every earlier
+    // tile in turn (1M keys, 256 tiles: ~8.7 us per pass).  A tile waits only
+    // on tiles with earlier tickets.
+    int G = 1;
+    while (G * G < ntiles) ++G;
+    const int g = (int)tile / G, first = g * G;
+    unsigned local[D];
 #pragma unroll
-    for (int e = 0; e < D; ++e) store_agent(row + e, tag | kFlagInc | total.v[e]);
-  } else {
+    for (int e = 0; e < D; ++e) local[e] = 0;
+    sum_published<D, RADIX>(row + (size_t)first * RADIX, (int)tile - first, local, epoch, ss);
+    unsigned long long* groups = row + (size_t)max_tiles * RADIX;
+    if ((int)tile == first + G - 1 && (int)tile < ntiles - 1)
+#pragma unroll
+      for (int e = 0; e < D; ++e)
+        store_agent(groups + (size_t)g * RADIX + e, tag | kFlagAgg | (local[e] + total.v[e]));
+    sum_published<D, RADIX>(groups, g, excl, epoch, ss);
+#pragma unroll
+    for (int e = 0; e < D; ++e) excl[e] += local[e];
+    // More tiles: decoupled look-back over earlier tiles for this thread's
+    // digits, LB tiles at a time: the window's words are loaded together,
+    // then summed from the nearest tile back to the first inclusive count or
+    // to the first word not yet published (read again on the next round).
+  } else if (tile != 0) {  // tile 0 published its inclusive counts above
     unsigned long long* mine = row + (size_t)tile * RADIX;
-#pragma unroll
-    for (int e = 0; e < D; ++e) store_agent(mine + e, tag | kFlagAgg | total.v[e]);
     int next[D];  // nearest earlier tile not yet summed, per digit
 #pragma unroll
     for (int e = 0; e < D; ++e) next[e] = (int)tile - 1;
@@ -651,9 +732,10 @@ __device__ __forceinline__ int2 local_up(int4 u, int id, int lo, int hi, int nm1
 }
 
 __global__ __launch_bounds__(kComThreads) void com_local_kernel(
-    const float4* __restrict__ pos_sorted, int n, Node* nodes, const int4* __restrict__ up,
+    float4* __restrict__ pos_sorted, int n, Node* nodes, const int4* __restrict__ up,
     const unsigned* __restrict__ visit, float4* box_lo, float4* box_hi, FrontEntry* seg1,
-    unsigned* cnt1, int2* rec1, unsigned* rcnt1, int* error) {
+    unsigned* cnt1, int2* rec1, unsigned* rcnt1, const unsigned* __restrict__ perm,
+    const float4* __restrict__ gsrc, int* error) {
   __shared__ float4 s_leaf[kComLeaves];
   __shared__ float4 s_com[kComLeaves];
   __shared__ float3 s_lo[kComLeaves], s_hi[kComLeaves];
@@ -678,7 +760,10 @@ __global__ __launch_bounds__(kComThreads) void com_local_kernel(
   bool has_rec = false;  // this thread's internal node belongs to a later pass
   int2 rec = make_int2(0, 0);
   if (lo + t <= hi) {
-    const float4 p = pos_sorted[lo + t];
+    // with gsrc this pass also gathers the sorted positions (pos_sorted is
+    // written here, read by the later passes and the walk)
+    const float4 p = gsrc ? gsrc[perm[lo + t]] : pos_sorted[lo + t];
+    if (gsrc) pos_sorted[lo + t] = p;
     const int4 ul = up[nm1 + lo + t];
     int4 un = make_int4(kUpRoot, 0, 0, 0);
     unsigned pass = 0;
@@ -1666,9 +1751,14 @@ int com_scratch_ints(int n) {
   return (int)ints;
 }
 
-void launch_com(hipStream_t s, const float4* pos_sorted, int n, Node* nodes, const int4* up,
-                unsigned* visit, float4* box_lo, float4* box_hi, int* scratch, int* error) {
-  if (n < 2) return;
+void launch_com(hipStream_t s, float4* pos_sorted, int n, Node* nodes, const int4* up,
+                unsigned* visit, float4* box_lo, float4* box_hi, int* scratch, int* error,
+                const unsigned* perm, const float4* gather_src) {
+  if (n < 2) {  // no tree; a single particle is its own sorted position
+    if (n == 1 && gather_src)
+      NB_HIP(hipMemcpyAsync(pos_sorted, gather_src, sizeof(float4), hipMemcpyDeviceToDevice, s));
+    return;
+  }
   const int g1 = com_g1(n), g2 = com_g2(n);
   if (g2 > kTopSegs) throw Error(kErrInvalid, "particle count too large for the COM top pass");
   static_assert(kComCounterInts % 4 == 0 && sizeof(FrontEntry) == 48, "16-byte aligned carve");
@@ -1681,7 +1771,7 @@ void launch_com(hipStream_t s, const float4* pos_sorted, int n, Node* nodes, con
   unsigned* rcnt1 = cnt1 + g1;
   unsigned* cnt2 = rcnt1 + g1;
   com_local_kernel<<<g1, kComThreads, 0, s>>>(pos_sorted, n, nodes, up, visit, box_lo, box_hi,
-                                              seg1, cnt1, rec1, rcnt1, error);
+                                              seg1, cnt1, rec1, rcnt1, perm, gather_src, error);
   NB_LAUNCH_CHECK();
   if (g1 == 1) return;
   com_mid_kernel<<<g2, kComThreads, 0, s>>>(pos_sorted, n, nodes, up, box_lo, box_hi, seg1, cnt1,

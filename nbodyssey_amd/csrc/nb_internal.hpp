@@ -77,6 +77,9 @@ constexpr int kMaxPasses = 8;
 constexpr int kSortBits = 8;
 // histogram words (pass p at p << bits), and look-back words per tile
 constexpr int kHistWords = kMaxPasses << kSortBits;
+// 64-bit look-back words of a sort buffer sized for max_tiles tiles: per pass,
+// max_tiles rows of tile aggregates and max_tiles rows of group totals.
+inline size_t lookback_words(int max_tiles) { return (size_t)2 * kHistWords * max_tiles; }
 constexpr int sort_bits(int key_bits) { return key_bits > 0 ? kSortBits : 0; }
 constexpr int sort_passes(int key_bits) { return key_bits == 30 ? 4 : 8; }
 // The buffer a sort reads its unsorted keys from: with an odd pass count the
@@ -215,8 +218,11 @@ __device__ __forceinline__ unsigned com_visit_word(int first, int last) {
                         : first / kComSuper == last / kComSuper ? 1u : 2u;
   return pass << 30;
 }
-void launch_com(hipStream_t s, const float4* pos_sorted, int n, Node* nodes, const int4* up,
-                unsigned* visit, float4* box_lo, float4* box_hi, int* scratch, int* error);
+// With gather_src, the first pass also writes pos_sorted[i] =
+// gather_src[perm[i]] (the sort then runs without its gather).
+void launch_com(hipStream_t s, float4* pos_sorted, int n, Node* nodes, const int4* up,
+                unsigned* visit, float4* box_lo, float4* box_hi, int* scratch, int* error,
+                const unsigned* perm = nullptr, const float4* gather_src = nullptr);
 
 // k_traverse.hip
 // Each kernel translation unit's code object is loaded by the runtime at its
@@ -301,7 +307,7 @@ struct BhScratch {
   float4* bbox_partial; // 2 * kBboxBlocks
   QuantParams* quant;
   SortState* state;     // reset by launch_sort_init before the first pass
-  unsigned long long* lookback;  // kHistWords * max_tiles
+  unsigned long long* lookback;  // lookback_words(max_tiles)
   int max_tiles;
 };
 // The buffers for n particles carved out of one block at `base` in 256-byte

@@ -55,7 +55,7 @@ const BhScratch& device_buffers(int dev, hipStream_t s) {
   launch_fill_u32(s, reinterpret_cast<unsigned*>(b.state), 0u, sizeof(SortState) / 4);
   launch_fill_u32(s, reinterpret_cast<unsigned*>(b.frontier), 0u, kComCounterInts);
   launch_fill_u32(s, reinterpret_cast<unsigned*>(b.lookback), 0u,
-                  (size_t)kHistWords * b.max_tiles * 2);
+                  lookback_words(b.max_tiles) * 2);
   g_bufs[dev] = b;
   g_ready[dev] = true;
   return g_bufs[dev];
