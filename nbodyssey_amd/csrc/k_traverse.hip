@@ -422,7 +422,6 @@ __device__ __forceinline__ void walk_group64(
   const int t = t0 + (live ? lane : count - 1);
   const int nm1 = n - 1;
   const float4 b = pos_sorted[t];
-  const unsigned long long lt = (1ull << lane) - 1ull;
   const unsigned long long live_mask = __builtin_amdgcn_ballot_w64(live);
 
   // bounding box of the live targets (dead lanes repeat the last target)
@@ -462,81 +461,86 @@ __device__ __forceinline__ void walk_group64(
   const unsigned nodes_total = 2u * (unsigned)n - 1u;
   unsigned pops = 0;
   if (lane == 0) stk[0] = root;
+  // exact-vote staging: the free upper half of the interaction list (fewer
+  // than 64 entries are pending between flushes)
+  float4* vbuf = list + 64;
   while (sp > 0) {
     // pop up to 64 entries; near the top of the stack fall back to one at a
     // time (a depth-first walk needs at most one entry per tree level, and
     // the reserve exceeds the deepest tree), so the stack cannot overflow
     int k = sp < 64 ? sp : 64;
     if (sp > WSTACK - kStackReserve) k = 1;
-    // pop k entries from the top: lane l takes stk[sp-1-l]
-    const bool valid = lane < k;
-    const int id = valid ? stk[sp - 1 - lane] : 0;
+    // lane l < k takes stk[sp-1-l]; the other lanes re-read the last popped
+    // entry (a valid id, so every load below stays in bounds) and are masked
+    // off in every ballot
+    const unsigned long long vmask = k >= 64 ? ~0ull : ((1ull << k) - 1ull);
+    const int id = stk[max(sp - 1 - lane, sp - k)];
     sp -= k;
     pops += k;  // a walk over a valid tree pops each node at most once
-    if (__ballot(valid && (unsigned)id >= nodes_total) || pops > nodes_total) {
+    if ((__builtin_amdgcn_ballot_w64((unsigned)id >= nodes_total) & vmask) ||
+        pops > nodes_total) {
       if (lane == 0) *error = 1;
       break;
     }
-    const bool is_leaf = valid && id >= nm1;
-    const bool is_node = valid && id < nm1;
-    // (x, y, z, m) of the leaf or node and the node's (left, right, size2):
-    // two loads issued together (with the leaf and node loads in separate
-    // branches the leaf load waited for the node load's registers)
-    const float4* srcp = is_leaf ? pos_sorted + (id - nm1) : &nodes[is_node ? id : 0].com;
-    float4 src = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (valid) src = *srcp;
-    int4 meta = make_int4(0, 0, 0, 0);
-    if (is_node) meta = *reinterpret_cast<const int4*>(&nodes[id].left);
+    const bool is_leaf = id >= nm1;
+    const unsigned long long leaf_mask = __builtin_amdgcn_ballot_w64(is_leaf) & vmask;
+    const unsigned long long node_mask = vmask & ~leaf_mask;
+    // (x, y, z, m) of the leaf or node and the node's (left, right, size2),
+    // loaded unconditionally (a leaf lane reads node 0's links, unused)
+    const int nid = is_leaf ? 0 : id;
+    const float4* srcp = is_leaf ? pos_sorted + (id - nm1) : &nodes[nid].com;
+    const float4 src = *srcp;
+    const int4 meta = *reinterpret_cast<const int4*>(&nodes[nid].left);
     const float s2 = __int_as_float(meta.z);
-    const int left = meta.x, right = meta.y;
-    bool sure_open = false, ambiguous = false, safe16 = false;
-    if (is_node) {
-      const float ex = fmaxf(fmaxf(blx - src.x, src.x - bhx), 0.f);
-      const float ey = fmaxf(fmaxf(bly - src.y, src.y - bhy), 0.f);
-      const float ez = fmaxf(fmaxf(blz - src.z, src.z - bhz), 0.f);
-      const float dmin2 = ex * ex + ey * ey + ez * ez;
-      const bool sure_accept = s2 < acc_lo * dmin2;
-      // all representative tests are evaluated (no short-circuit branches):
-      // open if one of them has theta2*d2 <= s2, i.e. the smallest d2 decides
-      float rmin = dist2(src.x - rx[0], src.y - ry[0], src.z - rz[0]);
+    // prefilter on every lane; only internal nodes' results are used
+    const float ex = fmaxf(fmaxf(blx - src.x, src.x - bhx), 0.f);
+    const float ey = fmaxf(fmaxf(bly - src.y, src.y - bhy), 0.f);
+    const float ez = fmaxf(fmaxf(blz - src.z, src.z - bhz), 0.f);
+    const float dmin2 = ex * ex + ey * ey + ez * ez;
+    const bool sure_accept = s2 < acc_lo * dmin2;
+    // all representative tests are evaluated (no short-circuit branches):
+    // open if one of them has theta2*d2 <= s2, i.e. the smallest d2 decides
+    float rmin = dist2(src.x - rx[0], src.y - ry[0], src.z - rz[0]);
 #pragma unroll
-      for (int r = 1; r < kReps; ++r)
-        rmin = fminf(rmin, dist2(src.x - rx[r], src.y - ry[r], src.z - rz[r]));
-      sure_open = !(s2 < theta2 * rmin);
-      ambiguous = !sure_accept & !sure_open;
-      if (HALF)
-        safe16 = (src.w <= kHalfMaxAccel * (dmin2 + eps2)) && (src.w <= kHalfMaxMass) &&
-                 (src.w >= kHalfMinMass) && (fabsf(src.x - gcx) < rmx) &&
-                 (fabsf(src.y - gcy) < rmy) && (fabsf(src.z - gcz) < rmz);
-    }
-    unsigned long long open_mask = __builtin_amdgcn_ballot_w64(sure_open);
-    unsigned long long amb = __builtin_amdgcn_ballot_w64(ambiguous);
+    for (int r = 1; r < kReps; ++r)
+      rmin = fminf(rmin, dist2(src.x - rx[r], src.y - ry[r], src.z - rz[r]));
+    const bool sure_open = !(s2 < theta2 * rmin);
+    unsigned long long open_mask = __builtin_amdgcn_ballot_w64(sure_open) & node_mask;
+    const unsigned long long amb =
+        node_mask & ~open_mask & ~__builtin_amdgcn_ballot_w64(sure_accept);
     // exact vote for the nodes the prefilter could not settle: their (com,
-    // size2) are staged in the free upper half of the interaction list
-    // (fewer than 64 entries are pending between flushes) and read back by
-    // LDS broadcast, two nodes per iteration
+    // size2) are staged in LDS, padded to a multiple of four with entries
+    // that never open (size2 = -1), and read back by LDS broadcast, four
+    // nodes per iteration
     if (amb) {
-      float4* vbuf = list + 64;
-      const int j = __popcll(amb & lt);  // this lane's rank among the ambiguous
-      if (ambiguous) vbuf[j] = make_float4(src.x, src.y, src.z, s2);
+      const int j = mbcnt64(amb);  // an ambiguous lane's rank among the ambiguous
       const int na = __popcll(amb);
-      unsigned long long opened = 0;  // bit k: the k-th ambiguous node opens
-      for (int k = 0; k < na; k += 2) {
-        const float4 v0 = vbuf[k];
-        const float4 v1 = vbuf[k + 1 < na ? k + 1 : k];
-        const bool op0 = !(v0.w < theta2 * dist2(v0.x - b.x, v0.y - b.y, v0.z - b.z));
-        const bool op1 = !(v1.w < theta2 * dist2(v1.x - b.x, v1.y - b.y, v1.z - b.z));
-        if (__builtin_amdgcn_ballot_w64(op0) & live_mask) opened |= 1ull << k;
-        if (__builtin_amdgcn_ballot_w64(op1) & live_mask) opened |= 2ull << k;
+      const int na4 = (na + 3) & ~3;
+      if (__builtin_amdgcn_inverse_ballot_w64(amb)) vbuf[j] = make_float4(src.x, src.y, src.z, s2);
+      if (lane < na4 - na) vbuf[na + lane] = make_float4(0.f, 0.f, 0.f, -1.f);
+      unsigned long long opened = 0;  // bit q: the q-th ambiguous node opens
+      for (int q = 0; q < na4; q += 4) {
+        const float4 v0 = vbuf[q], v1 = vbuf[q + 1], v2 = vbuf[q + 2], v3 = vbuf[q + 3];
+        const bool o0 = !(v0.w < theta2 * dist2(v0.x - b.x, v0.y - b.y, v0.z - b.z));
+        const bool o1 = !(v1.w < theta2 * dist2(v1.x - b.x, v1.y - b.y, v1.z - b.z));
+        const bool o2 = !(v2.w < theta2 * dist2(v2.x - b.x, v2.y - b.y, v2.z - b.z));
+        const bool o3 = !(v3.w < theta2 * dist2(v3.x - b.x, v3.y - b.y, v3.z - b.z));
+        if (__builtin_amdgcn_ballot_w64(o0) & live_mask) opened |= 1ull << q;
+        if (__builtin_amdgcn_ballot_w64(o1) & live_mask) opened |= 2ull << q;
+        if (__builtin_amdgcn_ballot_w64(o2) & live_mask) opened |= 4ull << q;
+        if (__builtin_amdgcn_ballot_w64(o3) & live_mask) opened |= 8ull << q;
       }
-      open_mask |= __builtin_amdgcn_ballot_w64(ambiguous && ((opened >> j) & 1ull));
+      open_mask |= __builtin_amdgcn_ballot_w64((opened >> j) & 1ull) & amb;
     }
-    const unsigned long long node_mask = __builtin_amdgcn_ballot_w64(is_node);
-    const unsigned long long all_src =
-        (node_mask & ~open_mask) | __builtin_amdgcn_ballot_w64(is_leaf);
+    const unsigned long long all_src = (node_mask & ~open_mask) | leaf_mask;
     // FP16 far field: accepted cells in range; everything else stays FP32
-    const unsigned long long far_mask =
-        HALF ? (node_mask & ~open_mask & __builtin_amdgcn_ballot_w64(safe16)) : 0ull;
+    unsigned long long far_mask = 0ull;
+    if (HALF) {
+      const bool safe16 = (src.w <= kHalfMaxAccel * (dmin2 + eps2)) && (src.w <= kHalfMaxMass) &&
+                          (src.w >= kHalfMinMass) && (fabsf(src.x - gcx) < rmx) &&
+                          (fabsf(src.y - gcy) < rmy) && (fabsf(src.z - gcz) < rmz);
+      far_mask = node_mask & ~open_mask & __builtin_amdgcn_ballot_w64(safe16);
+    }
     const unsigned long long src_mask = all_src & ~far_mask;
     if (COUNT) {
       ni += __popcll(all_src);
@@ -548,18 +552,18 @@ __device__ __forceinline__ void walk_group64(
       if (lane == 0) *overflow = 1;
       break;
     }
-    if ((open_mask >> lane) & 1ull) {
-      const int slot = sp + 2 * __popcll(open_mask & lt);
-      stk[slot] = left;
-      stk[slot + 1] = right;
+    if (__builtin_amdgcn_inverse_ballot_w64(open_mask)) {
+      const int ps = sp + 2 * mbcnt64(open_mask);
+      stk[ps] = meta.x;
+      stk[ps + 1] = meta.y;
     }
     sp += 2 * nopen;
     // append sources to the interaction list (lane order)
-    if ((src_mask >> lane) & 1ull) list[nlist + __popcll(src_mask & lt)] = src;
+    if (__builtin_amdgcn_inverse_ballot_w64(src_mask)) list[nlist + mbcnt64(src_mask)] = src;
     nlist += __popcll(src_mask);
     if (HALF) {
-      if ((far_mask >> lane) & 1ull) {
-        const int j = n16 + __popcll(far_mask & lt);
+      if (__builtin_amdgcn_inverse_ballot_w64(far_mask)) {
+        const int j = n16 + mbcnt64(far_mask);
         _Float16* e = reinterpret_cast<_Float16*>(l16 + 4 * (j >> 1)) + (j & 1);
         e[0] = (_Float16)(src.x - gcx);
         e[2] = (_Float16)(src.y - gcy);

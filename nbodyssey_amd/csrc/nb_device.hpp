@@ -71,6 +71,12 @@ __device__ __forceinline__ void kahan_add(float& sum, float& c, float x) {
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
+// Number of set bits of the 64-bit mask m below this lane (two VALU).
+__device__ __forceinline__ int mbcnt64(unsigned long long m) {
+  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_min(T v) {
 #pragma unroll
