@@ -98,7 +98,12 @@ constexpr int kTargetStack = 768;
 constexpr int kStackReserve = 64 + 2 * kStackDepth;
 constexpr int kWaveList = 128;     // LDS interaction-list entries per wave
 constexpr float kMacMargin = 1e-5f;
-constexpr int kReps = 8;           // representative targets for the open shortcut
+// Representative targets for the open shortcut: 8 per wave for the split
+// walk (4 or 2 per subgroup), 6 for the 64-wide group walk (1M two-disk,
+// theta 0.5: 6 -> 0.626 ms per walk, 8 -> 0.636 ms; each one costs six VALU
+// per batch, and the exact vote it saves costs about eight per node).
+constexpr int kReps = 8;
+constexpr int kGroupReps = 6;
 
 // POT: also accumulate the potential sum of m / sqrt(r^2 + eps^2) into pot
 // (m * inv is the first product of the force factor, so the force bits are
@@ -446,10 +451,10 @@ __device__ __forceinline__ void walk_group64(
   // in space); if one of them opens a node by its own canonical test, the
   // node is open -- an exact shortcut, no margin needed.  A dead lane holds
   // pos_sorted[n-1], a live member of this (last) group, so it is valid too.
-  float rx[kReps], ry[kReps], rz[kReps];
+  float rx[kGroupReps], ry[kGroupReps], rz[kGroupReps];
 #pragma unroll
-  for (int r = 0; r < kReps; ++r) {
-    const int src_lane = r * 63 / (kReps - 1);
+  for (int r = 0; r < kGroupReps; ++r) {
+    const int src_lane = r * 63 / (kGroupReps - 1);
     rx[r] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b.x), src_lane));
     ry[r] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b.y), src_lane));
     rz[r] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b.z), src_lane));
@@ -502,24 +507,22 @@ __device__ __forceinline__ void walk_group64(
     // open if one of them has theta2*d2 <= s2, i.e. the smallest d2 decides
     float rmin = dist2(src.x - rx[0], src.y - ry[0], src.z - rz[0]);
 #pragma unroll
-    for (int r = 1; r < kReps; ++r)
+    for (int r = 1; r < kGroupReps; ++r)
       rmin = fminf(rmin, dist2(src.x - rx[r], src.y - ry[r], src.z - rz[r]));
     const bool sure_open = !(s2 < theta2 * rmin);
     unsigned long long open_mask = __builtin_amdgcn_ballot_w64(sure_open) & node_mask;
     const unsigned long long amb =
         node_mask & ~open_mask & ~__builtin_amdgcn_ballot_w64(sure_accept);
     // exact vote for the nodes the prefilter could not settle: their (com,
-    // size2) are staged in LDS, padded to a multiple of four with entries
-    // that never open (size2 = -1), and read back by LDS broadcast, four
-    // nodes per iteration
+    // size2) are staged in LDS and read back by LDS broadcast, four nodes per
+    // iteration, then the remaining one to three one at a time
     if (amb) {
       const int j = mbcnt64(amb);  // an ambiguous lane's rank among the ambiguous
       const int na = __popcll(amb);
-      const int na4 = (na + 3) & ~3;
       if (__builtin_amdgcn_inverse_ballot_w64(amb)) vbuf[j] = make_float4(src.x, src.y, src.z, s2);
-      if (lane < na4 - na) vbuf[na + lane] = make_float4(0.f, 0.f, 0.f, -1.f);
       unsigned long long opened = 0;  // bit q: the q-th ambiguous node opens
-      for (int q = 0; q < na4; q += 4) {
+      int q = 0;
+      for (; q + 4 <= na; q += 4) {
         const float4 v0 = vbuf[q], v1 = vbuf[q + 1], v2 = vbuf[q + 2], v3 = vbuf[q + 3];
         const bool o0 = !(v0.w < theta2 * dist2(v0.x - b.x, v0.y - b.y, v0.z - b.z));
         const bool o1 = !(v1.w < theta2 * dist2(v1.x - b.x, v1.y - b.y, v1.z - b.z));
@@ -529,6 +532,11 @@ __device__ __forceinline__ void walk_group64(
         if (__builtin_amdgcn_ballot_w64(o1) & live_mask) opened |= 2ull << q;
         if (__builtin_amdgcn_ballot_w64(o2) & live_mask) opened |= 4ull << q;
         if (__builtin_amdgcn_ballot_w64(o3) & live_mask) opened |= 8ull << q;
+      }
+      for (; q < na; ++q) {
+        const float4 v = vbuf[q];
+        const bool o = !(v.w < theta2 * dist2(v.x - b.x, v.y - b.y, v.z - b.z));
+        if (__builtin_amdgcn_ballot_w64(o) & live_mask) opened |= 1ull << q;
       }
       open_mask |= __builtin_amdgcn_ballot_w64((opened >> j) & 1ull) & amb;
     }
