@@ -206,9 +206,21 @@ __global__ __launch_bounds__(kBlock) void ens_karras_kernel(
   const int right = (split + 1 == last) ? (n - 1 + split + 1) : split + 1;
   *reinterpret_cast<int2*>(&nodes[i].left) = make_int2(left, right);
   nodes[i].other = j;
-  up[left] = make_int4(i, right, j, 0);
-  up[right] = make_int4(i, left, j, 0);
-  visit[i] = com_visit_word(first, last);
+  // A node whose range holds more than one system joins whole systems (its
+  // split is a system boundary: the system bits differ there and nowhere
+  // inside a system).  The walks start at the system roots and never read
+  // such a node, so it gets no COM: its children's climbs end there as at a
+  // root, and its pass word (3) keeps it out of every COM pass.
+  const bool joins = (uint32_t)(kf >> 30) != (uint32_t)(keys[last] >> 30);
+  if (joins) {
+    up[left] = make_int4(-1, -1, j, 0);
+    up[right] = make_int4(-1, -1, j, 0);
+    visit[i] = 3u << 30;
+  } else {
+    up[left] = make_int4(i, right, j, 0);
+    up[right] = make_int4(i, left, j, 0);
+    visit[i] = com_visit_word(first, last);
+  }
   if (i == 0) up[0] = make_int4(-1, -1, j, 0);
   // the node covering exactly one system is that system's subtree root
   const int s = (int)(ki >> 30);

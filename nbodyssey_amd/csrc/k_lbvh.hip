@@ -774,7 +774,7 @@ __global__ __launch_bounds__(kComThreads) void com_local_kernel(
     s_leaf[t] = p;
     leaf_up = local_up(ul, nm1 + lo + t, lo, hi, nm1, total, error);
     if (lo + t < nm1) s_up[t] = local_up(un, lo + t, lo, hi, nm1, total, error);
-    if (pass != 0) {
+    if (pass == 1 || pass == 2) {  // 3: a node that gets no COM (ensemble forests)
       has_rec = true;
       rec = make_int2((int)((unsigned)(lo + t) | (pass << 30)), make_link(un.x, lo + t, un.y));
     }

@@ -212,7 +212,8 @@ constexpr int kTopList = 4096;    // listed pass-3 nodes
 int com_scratch_ints(int n);
 constexpr int kComCounterInts = 4;  // words at the start a new buffer must have zeroed
 // The `visit` word of internal node i of leaf range [first, last]: its COM
-// pass (0, 1 or 2) in bits 30-31, arrival bits clear.
+// pass (0, 1 or 2) in bits 30-31, arrival bits clear.  Pass word 3 marks a
+// node that gets no COM (an ensemble forest's nodes above the system roots).
 __device__ __forceinline__ unsigned com_visit_word(int first, int last) {
   const unsigned pass = first / kComLeaves == last / kComLeaves ? 0u
                         : first / kComSuper == last / kComSuper ? 1u : 2u;
