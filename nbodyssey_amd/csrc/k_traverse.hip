@@ -232,7 +232,6 @@ __device__ __forceinline__ void walk_target64(
   const int t = t0 + (live ? lane : count - 1);
   const int nm1 = n - 1;
   const float4 b = pos_sorted[t];
-  const unsigned long long lt = (1ull << lane) - 1ull;
   const unsigned long long live_mask = __builtin_amdgcn_ballot_w64(live);
   const float blx = wave_min(b.x), bly = wave_min(b.y), blz = wave_min(b.z);
   const float bhx = wave_max(b.x), bhy = wave_max(b.y), bhz = wave_max(b.z);
@@ -257,14 +256,12 @@ __device__ __forceinline__ void walk_target64(
     // pair at a time, two entries per tree level, so it cannot overflow)
     int k = sp < 64 ? sp : 64;
     if (sp > kTargetStack - kStackReserve) k = 2;
+    // lane l < k takes stk[sp-1-l] and its pair's set; the other lanes re-read
+    // the last popped entry (in bounds) with an empty set
     const bool in_batch = lane < k;
-    int id = -1;
-    unsigned long long m = 0ull;
-    if (in_batch) {
-      const int slot = sp - 1 - lane;
-      id = stk[slot];
-      m = smask[slot >> 1];
-    }
+    const int slot = max(sp - 1 - lane, sp - k);
+    const int id = stk[slot];
+    const unsigned long long m = in_batch ? smask[slot >> 1] : 0ull;
     sp -= k;
     pops += k;  // every node is pushed once, by its parent, with one set
     const bool valid = in_batch && id >= 0;
@@ -275,52 +272,62 @@ __device__ __forceinline__ void walk_target64(
     }
     const bool is_leaf = valid && id >= nm1;
     const bool is_node = valid && id < nm1;
-    // (x, y, z, m) of the leaf or node and the node's (left, right, size2):
-    // two loads issued together (with the leaf and node loads in separate
-    // branches the leaf load waited for the node load's registers)
-    const float4* srcp = is_leaf ? pos_sorted + (id - nm1) : &nodes[is_node ? id : 0].com;
-    float4 src = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (valid) src = *srcp;
-    int4 meta = make_int4(0, 0, 0, 0);
-    if (is_node) meta = *reinterpret_cast<const int4*>(&nodes[id].left);
+    // (x, y, z, m) of the leaf or node and the node's (left, right, size2),
+    // loaded unconditionally (other lanes read node 0, unused)
+    const int nid = is_node ? id : 0;
+    const float4* srcp = is_leaf ? pos_sorted + (id - nm1) : &nodes[nid].com;
+    const float4 src = *srcp;
+    const int4 meta = *reinterpret_cast<const int4*>(&nodes[nid].left);
     const float s2 = __int_as_float(meta.z);
     const int left = meta.x, right = meta.y;
-    bool all_open = false, amb = false;
-    if (is_node) {
-      const float ex = fmaxf(fmaxf(blx - src.x, src.x - bhx), 0.f);
-      const float ey = fmaxf(fmaxf(bly - src.y, src.y - bhy), 0.f);
-      const float ez = fmaxf(fmaxf(blz - src.z, src.z - bhz), 0.f);
-      const float dmin2 = ex * ex + ey * ey + ez * ez;
-      const float fx = fmaxf(fabsf(src.x - blx), fabsf(src.x - bhx));
-      const float fy = fmaxf(fabsf(src.y - bly), fabsf(src.y - bhy));
-      const float fz = fmaxf(fabsf(src.z - blz), fabsf(src.z - bhz));
-      const float dmax2 = fx * fx + fy * fy + fz * fz;
-      const bool all_acc = s2 < acc_lo * dmin2;
-      all_open = s2 > acc_hi * dmax2;
-      amb = !(all_acc || all_open);
-    }
-    unsigned long long om = all_open ? m : 0ull;  // lanes of the set that open the node
+    const float ex = fmaxf(fmaxf(blx - src.x, src.x - bhx), 0.f);
+    const float ey = fmaxf(fmaxf(bly - src.y, src.y - bhy), 0.f);
+    const float ez = fmaxf(fmaxf(blz - src.z, src.z - bhz), 0.f);
+    const float dmin2 = ex * ex + ey * ey + ez * ez;
+    const float fx = fmaxf(fabsf(src.x - blx), fabsf(src.x - bhx));
+    const float fy = fmaxf(fabsf(src.y - bly), fabsf(src.y - bhy));
+    const float fz = fmaxf(fabsf(src.z - blz), fabsf(src.z - bhz));
+    const float dmax2 = fx * fx + fy * fy + fz * fz;
+    const bool all_acc = s2 < acc_lo * dmin2;
+    const bool all_open = s2 > acc_hi * dmax2;
+    const bool amb = is_node && !(all_acc || all_open);
+    unsigned long long om = is_node && all_open ? m : 0ull;  // lanes of the set that open it
     const unsigned long long ambm = __builtin_amdgcn_ballot_w64(amb);
     if (ambm) {
       // per-lane canonical tests of the unsettled nodes, staged in the free
-      // upper halves of the list (< 64 entries pending), two per iteration;
-      // the ballots go back through the free upper halves of the set words
+      // upper half of the list (< 64 entries pending), four per iteration and
+      // the rest one at a time; lane 0 stores the q-th node's ballot in the
+      // free upper halves of the set words, where the node's lane reads it
       float4* vbuf = list + 64;
-      const int j = __popcll(ambm & lt);
+      const int j = mbcnt64(ambm);
       if (amb) vbuf[j] = make_float4(src.x, src.y, src.z, s2);
       const int na = __popcll(ambm);
-      for (int kk = 0; kk < na; kk += 2) {
-        const float4 v0 = vbuf[kk];
-        const float4 v1 = vbuf[kk + 1 < na ? kk + 1 : kk];
-        const bool op0 = !(v0.w < theta2 * dist2(v0.x - b.x, v0.y - b.y, v0.z - b.z));
-        const bool op1 = !(v1.w < theta2 * dist2(v1.x - b.x, v1.y - b.y, v1.z - b.z));
-        const unsigned long long b0 = __builtin_amdgcn_ballot_w64(op0);
-        const unsigned long long b1 = __builtin_amdgcn_ballot_w64(op1);
+      int q = 0;
+      for (; q + 4 <= na; q += 4) {
+        const float4 v0 = vbuf[q], v1 = vbuf[q + 1], v2 = vbuf[q + 2], v3 = vbuf[q + 3];
+        const unsigned long long b0 = __builtin_amdgcn_ballot_w64(
+            !(v0.w < theta2 * dist2(v0.x - b.x, v0.y - b.y, v0.z - b.z)));
+        const unsigned long long b1 = __builtin_amdgcn_ballot_w64(
+            !(v1.w < theta2 * dist2(v1.x - b.x, v1.y - b.y, v1.z - b.z)));
+        const unsigned long long b2 = __builtin_amdgcn_ballot_w64(
+            !(v2.w < theta2 * dist2(v2.x - b.x, v2.y - b.y, v2.z - b.z)));
+        const unsigned long long b3 = __builtin_amdgcn_ballot_w64(
+            !(v3.w < theta2 * dist2(v3.x - b.x, v3.y - b.y, v3.z - b.z)));
+        if (lane == 0) {  // q is a multiple of 4: 16-byte aligned
+          *reinterpret_cast<uint4*>(mlo + 64 + q) =
+              make_uint4((unsigned)b0, (unsigned)b1, (unsigned)b2, (unsigned)b3);
+          *reinterpret_cast<uint4*>(mhi + 64 + q) =
+              make_uint4((unsigned)(b0 >> 32), (unsigned)(b1 >> 32), (unsigned)(b2 >> 32),
+                         (unsigned)(b3 >> 32));
+        }
+      }
+      for (; q < na; ++q) {
+        const float4 v = vbuf[q];
+        const unsigned long long bq = __builtin_amdgcn_ballot_w64(
+            !(v.w < theta2 * dist2(v.x - b.x, v.y - b.y, v.z - b.z)));
         if (lane == 0) {
-          mlo[64 + kk] = (unsigned)b0;
-          mhi[64 + kk] = (unsigned)(b0 >> 32);
-          mlo[65 + kk] = (unsigned)b1;  // kk + 1 < 64: na <= 64 and kk is even
-          mhi[65 + kk] = (unsigned)(b1 >> 32);
+          mlo[64 + q] = (unsigned)bq;
+          mhi[64 + q] = (unsigned)(bq >> 32);
         }
       }
       if (amb) om = (((unsigned long long)mhi[64 + j] << 32) | mlo[64 + j]) & m;
@@ -330,7 +337,7 @@ __device__ __forceinline__ void walk_target64(
       // opening tests: internal nodes of the batch whose set holds this lane
       const unsigned long long nm = __builtin_amdgcn_ballot_w64(is_node);
       if (is_node) {
-        const int r = __popcll(nm & lt);
+        const int r = mbcnt64(nm);
         mlo[64 + r] = (unsigned)m;
         mhi[64 + r] = (unsigned)(m >> 32);
       }
@@ -344,17 +351,17 @@ __device__ __forceinline__ void walk_target64(
       if (lane == 0) *overflow = 1;
       break;
     }
-    if (om) {
-      const int slot = sp + 2 * __popcll(pm & lt);
-      stk[slot] = left;
-      stk[slot + 1] = right;
-      smask[slot >> 1] = om;
+    if (__builtin_amdgcn_inverse_ballot_w64(pm)) {
+      const int ps = sp + 2 * mbcnt64(pm);
+      stk[ps] = left;
+      stk[ps + 1] = right;
+      smask[ps >> 1] = om;
     }
     sp += 2 * npush;
     // append accepted nodes and leaves with their sets (lane order)
     const unsigned long long apm = __builtin_amdgcn_ballot_w64(am != 0ull);
-    if (am) {
-      const int e = nlist + __popcll(apm & lt);
+    if (__builtin_amdgcn_inverse_ballot_w64(apm)) {
+      const int e = nlist + mbcnt64(apm);
       list[e] = src;
       mlo[e] = (unsigned)am;
       mhi[e] = (unsigned)(am >> 32);
@@ -388,8 +395,8 @@ __global__ __launch_bounds__(kBlock) void walk_target_kernel(
   __shared__ int s_stack[WAVES][kTargetStack];
   __shared__ unsigned long long s_smask[WAVES][kTargetStack / 2];
   __shared__ float4 s_list[WAVES][kWaveList];
-  __shared__ unsigned s_mlo[WAVES][kWaveList];
-  __shared__ unsigned s_mhi[WAVES][kWaveList];
+  __shared__ __align__(16) unsigned s_mlo[WAVES][kWaveList];
+  __shared__ __align__(16) unsigned s_mhi[WAVES][kWaveList];
   const int w = threadIdx.x >> 6;
   const int gbase = xcd_block() * blockDim.x + (threadIdx.x & ~63);
   if (gbase >= n) return;  // wave-uniform
