@@ -775,7 +775,6 @@ __device__ __forceinline__ void walk_split64(
   const int t = t0 + (live ? lane : count - 1);
   const int nm1 = n - 1;
   const float4 b = pos_sorted[t];
-  const unsigned long long lt = (1ull << lane) - 1ull;
   const unsigned long long live_mask = __builtin_amdgcn_ballot_w64(live);
 
   // subgroup boxes (xor offsets below W stay inside a subgroup; dead lanes
@@ -832,13 +831,12 @@ __device__ __forceinline__ void walk_split64(
     // the subgroups that opened its parent, so it is popped at most once
     int k = sp < 64 ? sp : 64;
     if (sp > kWaveStack - kStackReserve) k = 1;
+    // lane l < k takes stk[sp-1-l]; the other lanes re-read the last popped
+    // entry (in bounds) with no subgroup
     const bool valid = lane < k;
-    int id = 0;
-    unsigned m = 0;
-    if (valid) {
-      id = stk[sp - 1 - lane];
-      m = smask[sp - 1 - lane];
-    }
+    const int slot = max(sp - 1 - lane, sp - k);
+    const int id = stk[slot];
+    const unsigned m = valid ? (unsigned)smask[slot] : 0u;
     sp -= k;
     pops += k;
     if (__ballot(valid && (unsigned)id >= nodes_total) || pops > nodes_total) {
@@ -847,67 +845,76 @@ __device__ __forceinline__ void walk_split64(
     }
     const bool is_leaf = valid && id >= nm1;
     const bool is_node = valid && id < nm1;
-    // (x, y, z, m) of the leaf or node and the node's (left, right, size2):
-    // two loads issued together (with the leaf and node loads in separate
-    // branches the leaf load waited for the node load's registers)
-    const float4* srcp = is_leaf ? pos_sorted + (id - nm1) : &nodes[is_node ? id : 0].com;
-    float4 src = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (valid) src = *srcp;
-    int4 meta = make_int4(0, 0, 0, 0);
-    if (is_node) meta = *reinterpret_cast<const int4*>(&nodes[id].left);
+    // (x, y, z, m) of the leaf or node and the node's (left, right, size2),
+    // loaded unconditionally (other lanes read node 0, unused)
+    const int nid = is_node ? id : 0;
+    const float4* srcp = is_leaf ? pos_sorted + (id - nm1) : &nodes[nid].com;
+    const float4 src = *srcp;
+    const int4 meta = *reinterpret_cast<const int4*>(&nodes[nid].left);
     const float s2 = __int_as_float(meta.z);
     const int left = meta.x, right = meta.y;
-    // per-subgroup prefilter, as in walk_group64 with the subgroup's box and
-    // representatives
+    // per-subgroup prefilter on every lane, as in walk_group64 with the
+    // subgroup's box and representatives; only internal nodes' bits are kept
     unsigned open_bits = 0, amb_bits = 0;
-    if (is_node) {
 #pragma unroll
-      for (int s = 0; s < S; ++s) {
-        const float ex = fmaxf(fmaxf(blx[s] - src.x, src.x - bhx[s]), 0.f);
-        const float ey = fmaxf(fmaxf(bly[s] - src.y, src.y - bhy[s]), 0.f);
-        const float ez = fmaxf(fmaxf(blz[s] - src.z, src.z - bhz[s]), 0.f);
-        const float dmin2 = ex * ex + ey * ey + ez * ez;
-        float rmin = dist2(src.x - rx[s][0], src.y - ry[s][0], src.z - rz[s][0]);
+    for (int s = 0; s < S; ++s) {
+      const float ex = fmaxf(fmaxf(blx[s] - src.x, src.x - bhx[s]), 0.f);
+      const float ey = fmaxf(fmaxf(bly[s] - src.y, src.y - bhy[s]), 0.f);
+      const float ez = fmaxf(fmaxf(blz[s] - src.z, src.z - bhz[s]), 0.f);
+      const float dmin2 = ex * ex + ey * ey + ez * ez;
+      float rmin = dist2(src.x - rx[s][0], src.y - ry[s][0], src.z - rz[s][0]);
 #pragma unroll
-        for (int r = 1; r < R; ++r)
-          rmin = fminf(rmin, dist2(src.x - rx[s][r], src.y - ry[s][r], src.z - rz[s][r]));
-        const bool sure_accept = s2 < acc_lo * dmin2;
-        const bool sure_open = !(s2 < theta2 * rmin);
-        open_bits |= (unsigned)sure_open << s;
-        amb_bits |= (unsigned)(!sure_accept & !sure_open) << s;
-      }
-      open_bits &= m;
-      amb_bits &= m;
+      for (int r = 1; r < R; ++r)
+        rmin = fminf(rmin, dist2(src.x - rx[s][r], src.y - ry[s][r], src.z - rz[s][r]));
+      const bool sure_accept = s2 < acc_lo * dmin2;
+      const bool sure_open = !(s2 < theta2 * rmin);
+      open_bits |= (unsigned)sure_open << s;
+      amb_bits |= (unsigned)(!sure_accept & !sure_open) << s;
     }
+    const unsigned node_bits = is_node ? m : 0u;
+    open_bits &= node_bits;
+    amb_bits &= node_bits;
     // exact vote for the unsettled (node, subgroup) pairs: every lane tests
-    // each ambiguous node, and the ballot is split by subgroup
+    // each ambiguous node, four per iteration and then one at a time, and the
+    // ballot is split by subgroup
     const unsigned long long amb = __builtin_amdgcn_ballot_w64(amb_bits != 0u);
     if (amb) {
       float4* vbuf = lists + 64;  // free upper half of list 0 (< 64 entries pending)
-      const int j = __popcll(amb & lt);
+      const int j = mbcnt64(amb);
       if (amb_bits) vbuf[j] = make_float4(src.x, src.y, src.z, s2);
       const int na = __popcll(amb);
       unsigned long long opened[S];
 #pragma unroll
       for (int s = 0; s < S; ++s) opened[s] = 0;
-      for (int kk = 0; kk < na; kk += 2) {
-        const float4 v0 = vbuf[kk];
-        const float4 v1 = vbuf[kk + 1 < na ? kk + 1 : kk];
-        const bool op0 = !(v0.w < theta2 * dist2(v0.x - b.x, v0.y - b.y, v0.z - b.z));
-        const bool op1 = !(v1.w < theta2 * dist2(v1.x - b.x, v1.y - b.y, v1.z - b.z));
-        const unsigned long long b0 = __builtin_amdgcn_ballot_w64(op0) & live_mask;
-        const unsigned long long b1 = __builtin_amdgcn_ballot_w64(op1) & live_mask;
+      int q = 0;
+      for (; q + 4 <= na; q += 4) {
+        unsigned long long bq[4];
 #pragma unroll
-        for (int s = 0; s < S; ++s) {
-          if ((b0 >> (s * W)) & kSubLanes) opened[s] |= 1ull << kk;
-          if ((b1 >> (s * W)) & kSubLanes) opened[s] |= 2ull << kk;
+        for (int i = 0; i < 4; ++i) {
+          const float4 v = vbuf[q + i];
+          bq[i] = __builtin_amdgcn_ballot_w64(
+                      !(v.w < theta2 * dist2(v.x - b.x, v.y - b.y, v.z - b.z))) & live_mask;
         }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int s = 0; s < S; ++s)
+            if ((bq[i] >> (s * W)) & kSubLanes) opened[s] |= 1ull << (q + i);
+      }
+      for (; q < na; ++q) {
+        const float4 v = vbuf[q];
+        const unsigned long long bv = __builtin_amdgcn_ballot_w64(
+                                          !(v.w < theta2 * dist2(v.x - b.x, v.y - b.y, v.z - b.z))) &
+                                      live_mask;
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+          if ((bv >> (s * W)) & kSubLanes) opened[s] |= 1ull << q;
       }
 #pragma unroll
       for (int s = 0; s < S; ++s)
         open_bits |= (((amb_bits >> s) & (unsigned)(opened[s] >> j)) & 1u) << s;
     }
-    const unsigned acc_bits = is_leaf ? m : is_node ? (m & ~open_bits) : 0u;
+    const unsigned acc_bits = is_leaf ? m : (node_bits & ~open_bits);
     if (COUNT) {
 #pragma unroll
       for (int s = 0; s < S; ++s)
@@ -920,12 +927,12 @@ __device__ __forceinline__ void walk_split64(
       if (lane == 0) *overflow = 1;
       break;
     }
-    if (open_bits) {
-      const int slot = sp + 2 * __popcll(open_mask & lt);
-      stk[slot] = left;
-      stk[slot + 1] = right;
-      smask[slot] = (unsigned char)open_bits;
-      smask[slot + 1] = (unsigned char)open_bits;
+    if (__builtin_amdgcn_inverse_ballot_w64(open_mask)) {
+      const int ps = sp + 2 * mbcnt64(open_mask);
+      stk[ps] = left;
+      stk[ps + 1] = right;
+      smask[ps] = (unsigned char)open_bits;
+      smask[ps + 1] = (unsigned char)open_bits;
     }
     sp += 2 * nopen;
     // append each source to the list of every subgroup that accepts it
@@ -933,7 +940,7 @@ __device__ __forceinline__ void walk_split64(
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       const unsigned long long am = __builtin_amdgcn_ballot_w64((acc_bits >> s) & 1u);
-      if ((am >> lane) & 1ull) lists[s * kWaveList + nl[s] + __popcll(am & lt)] = src;
+      if (__builtin_amdgcn_inverse_ballot_w64(am)) lists[s * kWaveList + nl[s] + mbcnt64(am)] = src;
       nl[s] += __popcll(am);
       if (COUNT) ni[s] += __popcll(am);
       most = nl[s] > most ? nl[s] : most;
