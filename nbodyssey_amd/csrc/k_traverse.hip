@@ -37,9 +37,12 @@ namespace nb {
 namespace {
 
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
-// L2).  Renumber them so that XCD x runs the x-th contiguous eighth of the
-// Morton-ordered targets: neighbouring groups open nearly the same nodes, so
-// each XCD's L2 then holds one region's part of the tree instead of all of it.
+// L2).  Renumber them so that each XCD runs runs of kXcdChunk consecutive
+// workgroups (Morton-adjacent targets, dealt to the XCDs in turn):
+// neighbouring groups open nearly the same nodes, so an XCD's L2 serves a run
+// from one region's part of the tree.  Runs of 4-64 workgroups measure the
+// same; whole eighths of the targets per XCD lose to the imbalance between
+// the XCDs' shares of the work.
 constexpr int kXcds = 8;
 constexpr int kXcdChunk = 16;  // consecutive workgroups kept on one XCD
 __device__ __forceinline__ int xcd_block() {
