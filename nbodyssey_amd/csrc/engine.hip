@@ -137,7 +137,7 @@ BhScratch carve_bh_scratch(void* base, int n, size_t* bytes) {
   s.vals_b = reinterpret_cast<unsigned*>(take(c * sizeof(unsigned)));
   s.pos_sorted = reinterpret_cast<float4*>(take(c * sizeof(float4)));
   s.nodes = reinterpret_cast<Node*>(take(c * sizeof(Node)));
-  s.up = reinterpret_cast<int4*>(take(2 * c * sizeof(int4)));
+  s.up = reinterpret_cast<int2*>(take(2 * c * sizeof(int2)));
   s.visit = reinterpret_cast<unsigned*>(take(c * sizeof(unsigned)));
   s.box_lo = reinterpret_cast<float4*>(take(c * sizeof(float4)));
   s.box_hi = reinterpret_cast<float4*>(take(c * sizeof(float4)));
@@ -909,7 +909,7 @@ class Engine {
   DevBuf<float4> pos_sorted, box_lo, box_hi, bh_partial;
   DevBuf<Node> nodes;
   DevBuf<int> frontier;
-  DevBuf<int4> up;
+  DevBuf<int2> up;
   DevBuf<QuantParams> quant;
   DevBuf<SortState> state;
   DevBuf<unsigned long long> lookback;
@@ -1387,9 +1387,9 @@ int nb_get_tree(nb_engine* h, uint64_t* keys_sorted, uint32_t* perm, int32_t* ch
     if (perm)
       NB_HIP(hipMemcpy(perm, e.perm_final(), n * sizeof(uint32_t), hipMemcpyDeviceToHost));
     if (parent) {
-      std::vector<int4> u(2 * (size_t)n - 1);
-      NB_HIP(hipMemcpy(u.data(), e.up.p, u.size() * sizeof(int4), hipMemcpyDeviceToHost));
-      for (size_t k = 0; k < u.size(); ++k) parent[k] = u[k].x;
+      std::vector<int2> u(2 * (size_t)n - 1);
+      NB_HIP(hipMemcpy(u.data(), e.up.p, u.size() * sizeof(int2), hipMemcpyDeviceToHost));
+      for (size_t k = 0; k < u.size(); ++k) parent[k] = nb::up_parent(u[k]);
     }
     if (pos_sorted4)
       NB_HIP(hipMemcpy(pos_sorted4, e.pos_sorted.p, n * sizeof(float4), hipMemcpyDeviceToHost));
@@ -1536,7 +1536,7 @@ int nb_karras_build(const uint64_t* keys_sorted, int n, int key_bits, int32_t* c
     nb::check_device();
     nb::DevBuf<uint64_t> dk;
     nb::DevBuf<nb::Node> nodes;
-    nb::DevBuf<int4> par;
+    nb::DevBuf<int2> par;
     nb::DevBuf<unsigned> visit;
     dk.alloc(n);
     if (key_bits == 30) {
@@ -1552,9 +1552,9 @@ int nb_karras_build(const uint64_t* keys_sorted, int n, int key_bits, int32_t* c
     nb::launch_karras(nullptr, key_bits, dk.p, n, nodes.p, par.p, visit.p, nullptr);
     NB_HIP(hipDeviceSynchronize());
     if (parent) {
-      std::vector<int4> u(2 * (size_t)n - 1);
-      NB_HIP(hipMemcpy(u.data(), par.p, u.size() * sizeof(int4), hipMemcpyDeviceToHost));
-      for (size_t k = 0; k < u.size(); ++k) parent[k] = u[k].x;
+      std::vector<int2> u(2 * (size_t)n - 1);
+      NB_HIP(hipMemcpy(u.data(), par.p, u.size() * sizeof(int2), hipMemcpyDeviceToHost));
+      for (size_t k = 0; k < u.size(); ++k) parent[k] = nb::up_parent(u[k]);
     }
     if (children) {
       std::vector<nb::Node> nd(n - 1);

@@ -163,7 +163,7 @@ __device__ __forceinline__ int ens_delta(const uint64_t* __restrict__ keys, cons
 
 __global__ __launch_bounds__(kBlock) void ens_karras_kernel(
     const uint64_t* __restrict__ keys, int n, const int* __restrict__ off, int nsys,
-    Node* __restrict__ nodes, int4* __restrict__ up, unsigned* __restrict__ visit,
+    Node* __restrict__ nodes, int2* __restrict__ up, unsigned* __restrict__ visit,
     int* __restrict__ roots, SortState* ss, unsigned* __restrict__ box) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (blockIdx.x == 0) {
@@ -213,15 +213,16 @@ __global__ __launch_bounds__(kBlock) void ens_karras_kernel(
   // root, and its pass word (3) keeps it out of every COM pass.
   const bool joins = (uint32_t)(kf >> 30) != (uint32_t)(keys[last] >> 30);
   if (joins) {
-    up[left] = make_int4(-1, -1, j, 0);
-    up[right] = make_int4(-1, -1, j, 0);
+    up[left] = up_none();
+    up[right] = up_none();
     visit[i] = 3u << 30;
   } else {
-    up[left] = make_int4(i, right, j, 0);
-    up[right] = make_int4(i, left, j, 0);
-    visit[i] = com_visit_word(first, last);
+    const unsigned vw = com_visit_word(first, last);
+    up[left] = make_up(i, vw, right);
+    up[right] = make_up(i, vw, left);
+    visit[i] = vw;
   }
-  if (i == 0) up[0] = make_int4(-1, -1, j, 0);
+  if (i == 0) up[0] = up_none();
   // the node covering exactly one system is that system's subtree root
   const int s = (int)(ki >> 30);
   if (first == off[s] && last == off[s + 1] - 1) roots[s] = i;
@@ -346,7 +347,7 @@ void launch_ens_morton_hist(hipStream_t s, const float4* pos, int n, const int* 
 }
 
 void launch_ens_karras(hipStream_t s, const uint64_t* keys, int n, const int* off, int nsys,
-                       Node* nodes, int4* up, unsigned* visit, int* roots, SortState* ss,
+                       Node* nodes, int2* up, unsigned* visit, int* roots, SortState* ss,
                        unsigned* box) {
   if (n < 2) return;
   const int g = (n - 1 + kBlock - 1) / kBlock;

@@ -524,7 +524,7 @@ __device__ __forceinline__ int delta_w(const KeyWindow<K>& key, int n, int i, K 
 template <typename K>
 __global__ __launch_bounds__(kBlock) void karras_kernel(const K* __restrict__ keys, int n,
                                                         Node* __restrict__ nodes,
-                                                        int4* __restrict__ up,
+                                                        int2* __restrict__ up,
                                                         unsigned* __restrict__ visit,
                                                         SortState* ss) {
   __shared__ K s_key[kKarrasStage];
@@ -563,11 +563,12 @@ __global__ __launch_bounds__(kBlock) void karras_kernel(const K* __restrict__ ke
   int right = (split + 1 == last) ? (n - 1 + split + 1) : split + 1;
   *reinterpret_cast<int2*>(&nodes[i].left) = make_int2(left, right);
   nodes[i].other = j;
-  // each child's upward record: parent, sibling, far end of the parent's range
-  up[left] = make_int4(i, right, j, 0);
-  up[right] = make_int4(i, left, j, 0);
-  visit[i] = com_visit_word(first, last);
-  if (i == 0) up[0] = make_int4(-1, -1, j, 0);
+  // each child's upward record: parent with the parent's pass word, sibling
+  const unsigned vw = com_visit_word(first, last);
+  up[left] = make_up(i, vw, right);
+  up[right] = make_up(i, vw, left);
+  visit[i] = vw;
+  if (i == 0) up[0] = up_none();
 }
 
 // Node data a climbing thread carries: centre of mass (x, y, z, m) and AABB.
@@ -714,25 +715,27 @@ static_assert(kComLeaves == 1024, "list entries pack 11-bit child slots and 10-b
 // parent to the following round.
 constexpr int kClimb = 1;
 
-__device__ __forceinline__ int2 local_up(int4 u, int id, int lo, int hi, int nm1, int total,
+__device__ __forceinline__ int2 local_up(int2 u, int id, int lo, int hi, int nm1, int total,
                                          int* error) {
-  if (u.x < 0) return make_int2(kUpRoot, 0);
-  if (u.x >= nm1 || (unsigned)u.y >= (unsigned)total) {
+  const int parent = up_parent(u);
+  if (parent < 0) return make_int2(kUpRoot, 0);
+  if (parent >= nm1 || (unsigned)u.y >= (unsigned)total) {
     *error = 1;
     return make_int2(kUpRoot, 0);
   }
-  if (min(u.x, u.z) < lo || max(u.x, u.z) > hi) return make_int2(kUpOut, make_link(u.x, id, u.y));
-  // a local parent's children lie inside its range, so the sibling is local
+  // pass 0: the parent's leaf range lies in one pass-1 block, this child's
+  if (up_pass(u) != 0) return make_int2(kUpOut, make_link(parent, id, u.y));
+  // a local parent's children lie inside its range, so both are local
   const int sib = u.y >= nm1 ? u.y - nm1 : u.y;
-  if (sib < lo || sib > hi) {
+  if (parent < lo || parent > hi || sib < lo || sib > hi) {
     *error = 1;
     return make_int2(kUpRoot, 0);
   }
-  return make_int2(u.x - lo, u.y >= nm1 ? kComLeaves + sib - lo : sib - lo);
+  return make_int2(parent - lo, u.y >= nm1 ? kComLeaves + sib - lo : sib - lo);
 }
 
 __global__ __launch_bounds__(kComThreads) void com_local_kernel(
-    float4* __restrict__ pos_sorted, int n, Node* nodes, const int4* __restrict__ up,
+    float4* __restrict__ pos_sorted, int n, Node* nodes, const int2* __restrict__ up,
     const unsigned* __restrict__ visit, float4* box_lo, float4* box_hi, FrontEntry* seg1,
     unsigned* cnt1, int2* rec1, unsigned* rcnt1, const unsigned* __restrict__ perm,
     const float4* __restrict__ gsrc, int* error) {
@@ -764,8 +767,8 @@ __global__ __launch_bounds__(kComThreads) void com_local_kernel(
     // written here, read by the later passes and the walk)
     const float4 p = gsrc ? gsrc[perm[lo + t]] : pos_sorted[lo + t];
     if (gsrc) pos_sorted[lo + t] = p;
-    const int4 ul = up[nm1 + lo + t];
-    int4 un = make_int4(kUpRoot, 0, 0, 0);
+    const int2 ul = up[nm1 + lo + t];
+    int2 un = up_none();
     unsigned pass = 0;
     if (lo + t < nm1) {
       un = up[lo + t];
@@ -776,7 +779,8 @@ __global__ __launch_bounds__(kComThreads) void com_local_kernel(
     if (lo + t < nm1) s_up[t] = local_up(un, lo + t, lo, hi, nm1, total, error);
     if (pass == 1 || pass == 2) {  // 3: a node that gets no COM (ensemble forests)
       has_rec = true;
-      rec = make_int2((int)((unsigned)(lo + t) | (pass << 30)), make_link(un.x, lo + t, un.y));
+      rec = make_int2((int)((unsigned)(lo + t) | (pass << 30)),
+                      make_link(up_parent(un), lo + t, un.y));
     }
   }
   __syncthreads();
@@ -909,13 +913,14 @@ __global__ __launch_bounds__(kComThreads) void com_local_kernel(
 }
 
 // Where a climb goes from an upward record: 0 stop at the root, 1 malformed
-// record, 2 the parent leaves [lo, hi] (frontier), 3 climb to the parent.
-// Every field of the record is consumed here, before any branch, so the
-// record is fetched as one 16-byte load.
-__device__ __forceinline__ int climb_status(int4 u, int nm1, int total, int lo, int hi) {
-  const bool bad = u.x >= nm1 || (unsigned)u.y >= (unsigned)total;
-  const bool out = min(u.x, u.z) < lo || max(u.x, u.z) > hi;
-  return u.x < 0 ? 0 : bad ? 1 : out ? 2 : 3;
+// record, 2 the parent's pass is above max_pass (its leaf range leaves this
+// pass's superblock: a frontier entry), 3 climb to the parent.  Both words of
+// the record are consumed here, before any branch, so it is fetched as one
+// 8-byte load.
+__device__ __forceinline__ int climb_status(int2 u, int nm1, int total, unsigned max_pass) {
+  const bool bad = up_parent(u) >= nm1 || (unsigned)u.y >= (unsigned)total;
+  const bool out = up_pass(u) > max_pass;
+  return up_root(u) ? 0 : bad ? 1 : out ? 2 : 3;
 }
 
 // Climb shared by passes 2 and 3.  The two children of a node arrive through
@@ -945,9 +950,10 @@ constexpr int kMaxClimb = 128;
 
 template <typename Arrive>
 __device__ __forceinline__ void climb_exchange(const float4* __restrict__ pos_sorted, int n,
-                                               Node* nodes, const int4* __restrict__ up,
-                                               float4* box_lo, float4* box_hi, int child, int lo,
-                                               int hi, FrontEntry* seg, lds<unsigned>* s_cnt,
+                                               Node* nodes, const int2* __restrict__ up,
+                                               float4* box_lo, float4* box_hi, int child,
+                                               unsigned max_pass, FrontEntry* seg,
+                                               lds<unsigned>* s_cnt,
                                                const ComSlots& slots, Arrive arrive, int* error) {
   const int nm1 = n - 1;
   const int total = 2 * n - 1;
@@ -956,22 +962,22 @@ __device__ __forceinline__ void climb_exchange(const float4* __restrict__ pos_so
     return;
   }
   NodeData cur = load_data(pos_sorted, nm1, nodes, box_lo, box_hi, child);
-  int4 u = up[child];
+  int2 u = up[child];
   bool in_global = true;  // cur is also in global memory and visible (an earlier launch wrote it)
   for (int level = 0;; ++level) {
-    const int st = climb_status(u, nm1, total, lo, hi);
+    const int st = climb_status(u, nm1, total, max_pass);
     if (st == 0) break;
     if (st == 1 || level > kMaxClimb) {
       *error = 1;
       break;
     }
     if (st == 2) {  // frontier entry of the next pass (none in the last pass)
-      if (seg) push_entry(seg, s_cnt, cur, make_link(u.x, child, u.y), child, error);
+      if (seg) push_entry(seg, s_cnt, cur, make_link(up_parent(u), child, u.y), child, error);
       else *error = 1;
       break;
     }
-    const int node = u.x, sib = u.y;
-    const int4 next = up[node];
+    const int node = up_parent(u), sib = u.y;
+    const int2 next = up[node];
     const unsigned slot = __hip_atomic_fetch_add(slots.used, 1u, __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
     unsigned tag;
@@ -1230,7 +1236,7 @@ constexpr size_t kMidBytes = kMidStagedBytes > kMidUnstagedBytes ? kMidStagedByt
                                                                  : kMidUnstagedBytes;
 constexpr int kMidTops = 256;  // pass-3 nodes a superblock collects before listing them
 __global__ __launch_bounds__(kComThreads) void com_mid_kernel(
-    const float4* __restrict__ pos_sorted, int n, Node* nodes, const int4* __restrict__ up,
+    const float4* __restrict__ pos_sorted, int n, Node* nodes, const int2* __restrict__ up,
     float4* box_lo, float4* box_hi, const FrontEntry* __restrict__ seg1,
     const unsigned* __restrict__ cnt1, const int2* __restrict__ rec1,
     const unsigned* __restrict__ rcnt1, int nblocks1, FrontEntry* seg2, unsigned* cnt2,
@@ -1337,8 +1343,8 @@ __global__ __launch_bounds__(kComThreads) void com_mid_kernel(
     if (staged)
       arrive_entry(e, st, index, out, nfront, error);
     else
-      climb_exchange(pos_sorted, n, nodes, up, box_lo, box_hi, __float_as_int(e.hi.w), lo, hi,
-                     out, nfront, slots, arrive, error);
+      climb_exchange(pos_sorted, n, nodes, up, box_lo, box_hi, __float_as_int(e.hi.w), 1u, out,
+                     nfront, slots, arrive, error);
   }
   if (staged) {  // uniform: nmid is one value for the block
     __syncthreads();
@@ -1372,7 +1378,7 @@ static_assert(kTopHash == 1 << 11, "top_hash yields 11 bits");
 static_assert(kTopCap <= kComThreads, "one listed node per thread in the first load");
 
 __global__ __launch_bounds__(kComThreads) void com_top_kernel(
-    const float4* __restrict__ pos_sorted, int n, Node* nodes, const int4* __restrict__ up,
+    const float4* __restrict__ pos_sorted, int n, Node* nodes, const int2* __restrict__ up,
     unsigned* visit, float4* box_lo, float4* box_hi, const FrontEntry* __restrict__ seg2,
     const unsigned* __restrict__ cnt2, int nblocks2, TopList top, int* error) {
   __shared__ __align__(16) unsigned char s_raw[kTopBytes];
@@ -1462,8 +1468,8 @@ __global__ __launch_bounds__(kComThreads) void com_top_kernel(
       if (staged)
         arrive_entry(e, st, index, nullptr, nullptr, error);
       else
-        climb_exchange(pos_sorted, n, nodes, up, box_lo, box_hi, __float_as_int(e.hi.w), 0,
-                       n - 1, nullptr, nullptr, slots, arrive, error);
+        climb_exchange(pos_sorted, n, nodes, up, box_lo, box_hi, __float_as_int(e.hi.w), 2u,
+                       nullptr, nullptr, slots, arrive, error);
     }
   }
   if (staged) {  // uniform: listed is one value for the block
@@ -1728,7 +1734,7 @@ void launch_sort_u64(hipStream_t s, uint64_t* keys_a, uint64_t* keys_b, unsigned
                                   first_pass_identity, gather_src, gather_dst, passes);
 }
 
-void launch_karras(hipStream_t s, int key_bits, const void* keys, int n, Node* nodes, int4* up,
+void launch_karras(hipStream_t s, int key_bits, const void* keys, int n, Node* nodes, int2* up,
                    unsigned* visit, SortState* ss) {
   if (n < 2) return;
   int g = (n - 1 + kBlock - 1) / kBlock;
@@ -1751,7 +1757,7 @@ int com_scratch_ints(int n) {
   return (int)ints;
 }
 
-void launch_com(hipStream_t s, float4* pos_sorted, int n, Node* nodes, const int4* up,
+void launch_com(hipStream_t s, float4* pos_sorted, int n, Node* nodes, const int2* up,
                 unsigned* visit, float4* box_lo, float4* box_hi, int* scratch, int* error,
                 const unsigned* perm, const float4* gather_src) {
   if (n < 2) {  // no tree; a single particle is its own sorted position

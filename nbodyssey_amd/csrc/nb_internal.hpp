@@ -6,7 +6,7 @@
 //   vals_a/vals_b        u32 x n, original index riding with each key
 //   pos_sorted           float4 x n, positions gathered into Morton order
 //   nodes                Node (32 B) x (n-1), one record per internal node
-//   up                   int4 x (2n-1): parent, sibling, parent's range end (root: -1)
+//   up                   int2 x (2n-1): parent id | parent pass << 30, sibling (root: -1)
 //   visit                u32 x (n-1), bottom-up arrival counters
 //   box_lo/box_hi        float4 x (n-1), internal-node AABBs (used by the COM pass only)
 #pragma once
@@ -190,7 +190,7 @@ void launch_sort_u64(hipStream_t s, uint64_t* keys_a, uint64_t* keys_b, unsigned
                      const float4* gather_src, float4* gather_dst);
 // Also resets `ss` (histograms, tickets, epoch) for the next sort when ss is
 // not null.
-void launch_karras(hipStream_t s, int key_bits, const void* keys, int n, Node* nodes, int4* up,
+void launch_karras(hipStream_t s, int key_bits, const void* keys, int n, Node* nodes, int2* up,
                    unsigned* visit, SortState* ss);
 // COM passes (k_lbvh.hip).  Pass 1 combines the nodes whose leaf range lies
 // in one block of kComLeaves leaves, pass 2 those inside one superblock of
@@ -219,9 +219,22 @@ __device__ __forceinline__ unsigned com_visit_word(int first, int last) {
                         : first / kComSuper == last / kComSuper ? 1u : 2u;
   return pass << 30;
 }
+// Upward record of a node or leaf (id: leaves n - 1 + index), written by its
+// parent's Karras thread: x = parent id | the parent's `visit` pass word, or
+// -1 at a root; y = sibling id.  The parent's pass is all the COM passes ask
+// of its leaf range: pass 0 means it lies inside the child's pass-1 block,
+// pass <= 1 inside the child's superblock.  Pass 2 sets the sign bit, so a
+// root is x == -1 exactly (parent ids stay below 2^29, so no record is -1).
+__device__ __forceinline__ int2 make_up(int parent, unsigned visit_word, int sibling) {
+  return make_int2((int)((unsigned)parent | visit_word), sibling);
+}
+__device__ __forceinline__ int2 up_none() { return make_int2(-1, -1); }
+__host__ __device__ inline bool up_root(int2 u) { return u.x == -1; }
+__host__ __device__ inline int up_parent(int2 u) { return up_root(u) ? -1 : u.x & 0x3fffffff; }
+__device__ __forceinline__ unsigned up_pass(int2 u) { return (unsigned)u.x >> 30; }
 // With gather_src, the first pass also writes pos_sorted[i] =
 // gather_src[perm[i]] (the sort then runs without its gather).
-void launch_com(hipStream_t s, float4* pos_sorted, int n, Node* nodes, const int4* up,
+void launch_com(hipStream_t s, float4* pos_sorted, int n, Node* nodes, const int2* up,
                 unsigned* visit, float4* box_lo, float4* box_hi, int* scratch, int* error,
                 const unsigned* perm = nullptr, const float4* gather_src = nullptr);
 
@@ -283,7 +296,7 @@ void launch_ens_morton_hist(hipStream_t s, const float4* pos, int n, const int* 
 // boxes for the next step, and writes roots[s] = the internal node whose leaf
 // range is exactly system s.
 void launch_ens_karras(hipStream_t s, const uint64_t* keys, int n, const int* off, int nsys,
-                       Node* nodes, int4* up, unsigned* visit, int* roots, SortState* ss,
+                       Node* nodes, int2* up, unsigned* visit, int* roots, SortState* ss,
                        unsigned* box);
 // tiles: (system, first target) per 256-target tile; tile_off[s] = first tile
 // of system s; out: (total, kinetic, potential) per system.
@@ -300,7 +313,7 @@ struct BhScratch {
   unsigned* vals_b;
   float4* pos_sorted;   // n
   Node* nodes;          // n - 1
-  int4* up;             // 2n - 1
+  int2* up;             // 2n - 1
   unsigned* visit;      // n - 1
   float4* box_lo;       // n - 1
   float4* box_hi;       // n - 1
