@@ -307,7 +307,7 @@ class Engine {
                       state.p, sys_box.p);
     mark(ce, kPhBuild + 1);
     launch_com(stream, pos_sorted.p, n, nodes.p, up.p, visit.p, box_lo.p, box_hi.p, frontier.p,
-               &state.p->error);
+               &state.p->error, nullptr, nullptr, sys_off.p, nsys);
     mark(ce, kPhCom + 1);
     WalkArgs w = walk_args();
     w.vel = step ? vel : nullptr;

@@ -738,7 +738,7 @@ __global__ __launch_bounds__(kComThreads) void com_local_kernel(
     float4* __restrict__ pos_sorted, int n, Node* nodes, const int2* __restrict__ up,
     const unsigned* __restrict__ visit, float4* box_lo, float4* box_hi, FrontEntry* seg1,
     unsigned* cnt1, int2* rec1, unsigned* rcnt1, const unsigned* __restrict__ perm,
-    const float4* __restrict__ gsrc, int* error) {
+    const float4* __restrict__ gsrc, unsigned* top_count, int* error) {
   __shared__ float4 s_leaf[kComLeaves];
   __shared__ float4 s_com[kComLeaves];
   __shared__ float3 s_lo[kComLeaves], s_hi[kComLeaves];
@@ -755,6 +755,9 @@ __global__ __launch_bounds__(kComThreads) void com_local_kernel(
   FrontEntry* front = seg1 + (size_t)blockIdx.x * kSeg;
   if (t < kComLeaves / 16) s_visit[t] = 0;
   if (t < 3) s_cnt[t] = 0;
+  // pass 2 of this build appends to the pass-3 list; the last build's pass 3
+  // has read its count
+  if (blockIdx.x == 0 && t == 0) *top_count = 0;
   if (t == 0) {
     s_nfront = 0;
     s_nrec = 0;
@@ -1362,8 +1365,19 @@ __global__ __launch_bounds__(kComThreads) void com_mid_kernel(
 // first entry and list record are loaded at the start.  With more than
 // kTopCap listed nodes it climbs with arrival tags in the low bits of the
 // per-node `visit` words in global memory (written by the Karras kernel with
-// those bits clear; one workgroup, so workgroup scope holds) and exchange
-// slots instead.  Earlier passes are visible across the kernel boundaries.
+// those bits clear; each node's word is touched by one workgroup only, so
+// workgroup scope holds) and exchange slots instead.  Earlier passes are
+// visible across the kernel boundaries.
+//
+// A forest's systems are independent trees (the nodes above the system roots
+// get no COM), so its pass 3 runs as kTopGroups workgroups: workgroup w takes
+// systems [w S / W, (w + 1) S / W), i.e. the listed nodes whose ids lie in
+// those systems' leaf range (an internal node's id is one end of its leaf
+// range) and the entries whose parents do, from the superblocks that range
+// overlaps.  Each stages its own share (the 256-system ensemble lists ~2900
+// nodes, beyond one workgroup's kTopCap).  Pass 1 resets the list count for
+// the next build, when every workgroup here has read it.
+constexpr int kTopGroups = 16;
 constexpr int kTopCap = 1024;
 constexpr int kTopHash = 2048;  // power of two
 constexpr int kTopSlots = 2048;
@@ -1380,15 +1394,30 @@ static_assert(kTopCap <= kComThreads, "one listed node per thread in the first l
 __global__ __launch_bounds__(kComThreads) void com_top_kernel(
     const float4* __restrict__ pos_sorted, int n, Node* nodes, const int2* __restrict__ up,
     unsigned* visit, float4* box_lo, float4* box_hi, const FrontEntry* __restrict__ seg2,
-    const unsigned* __restrict__ cnt2, int nblocks2, TopList top, int* error) {
+    const unsigned* __restrict__ cnt2, int nblocks2, TopList top,
+    const int* __restrict__ sys_off, int nsys, int* error) {
   __shared__ __align__(16) unsigned char s_raw[kTopBytes];
-  __shared__ unsigned s_used, s_nready;
-  const unsigned listed = *top.count;
-  const bool staged = listed <= (unsigned)kTopCap;
+  __shared__ unsigned s_used, s_nready, s_nown;
+  const bool part = gridDim.x > 1;  // a forest split by systems (uniform)
+  // node ids [own_lo, own_hi) and superblocks [bfirst, blast] of this workgroup
+  int own_lo = 0, own_hi = n, bfirst = 0, blast = nblocks2 - 1;
+  if (part) {
+    const int s0 = (int)((long long)blockIdx.x * nsys / gridDim.x);
+    const int s1 = (int)((long long)(blockIdx.x + 1) * nsys / gridDim.x);
+    own_lo = sys_off[s0];
+    own_hi = sys_off[s1];
+    bfirst = own_lo / kComSuper;
+    blast = min(nblocks2 - 1, max(own_hi - 1, 0) / kComSuper);
+  }
+  auto owns = [&](int id) { return id >= own_lo && id < own_hi; };
+  const unsigned listed_all = *top.count;
+  // staging needs the whole list, and at most kTopCap nodes (known for one
+  // workgroup now, for a share of a forest after the selection below)
+  const bool may_stage = listed_all <= (unsigned)(part ? kTopList : kTopCap);
   // first loads: this thread's listed node and its first entry
   const int2 rec0 = top.list[threadIdx.x];
-  const int b0 = threadIdx.x >> 5, slot0 = threadIdx.x & 31;
-  const int bq = min(b0, nblocks2 - 1);
+  const int b0 = bfirst + (int)(threadIdx.x >> 5), slot0 = threadIdx.x & 31;
+  const int bq = max(0, min(b0, blast));
   const unsigned nent0_raw = cnt2[bq];
   const FrontEntry ent0 = seg2[(size_t)bq * kSeg + slot0];
   lds<unsigned char>* raw = (lds<unsigned char>*)s_raw;
@@ -1410,20 +1439,41 @@ __global__ __launch_bounds__(kComThreads) void com_top_kernel(
   if (threadIdx.x == 0) {
     s_used = 0;
     s_nready = 0;
+    s_nown = 0;
   }
-  if (staged)
+  if (may_stage)
     for (int k = threadIdx.x; k < kTopHash; k += blockDim.x) s_key[k] = -1;
   __syncthreads();
+  unsigned listed = listed_all;
+  int2 mine = rec0;  // the listed node thread k stages at index k
+  if (part) {
+    if (may_stage)
+      for (unsigned k = threadIdx.x; k < listed_all; k += blockDim.x) {
+        const int2 r = k == threadIdx.x ? rec0 : top.list[k];
+        if (!owns(r.x)) continue;
+        const unsigned ix = __hip_atomic_fetch_add(&s_nown, 1u, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (ix < (unsigned)kTopCap) {
+          st.id[ix] = r.x;
+          st.link[ix] = r.y;
+        }
+      }
+    __syncthreads();
+    listed = s_nown;
+    if (threadIdx.x < min(listed, (unsigned)kTopCap))
+      mine = make_int2(st.id[threadIdx.x], st.link[threadIdx.x]);
+  }
+  const bool staged = may_stage && listed <= (unsigned)kTopCap;
   if (staged && threadIdx.x < listed) {
     const int k = threadIdx.x;
-    const int id = rec0.x;
+    const int id = mine.x;
     if (id < 0 || id >= n - 1) {  // only on a stale or corrupt list
       *error = 1;
       st.id[k] = 0;  // keeps every later access in range
       st.link[k] = -1;
     } else {
       st.id[k] = id;
-      st.link[k] = rec0.y;
+      st.link[k] = mine.y;
       unsigned h = top_hash(id);
       bool placed = false;
       for (int probe = 0; probe < kTopHash && !placed; ++probe) {
@@ -1458,13 +1508,15 @@ __global__ __launch_bounds__(kComThreads) void com_top_kernel(
     return __hip_atomic_fetch_or(&visit[node], tag, __ATOMIC_RELAXED,
                                  __HIP_MEMORY_SCOPE_WORKGROUP) & 0xffffu;
   };
-  // the whole tree is in range here: an entry or climb whose parent is not
-  // listed means a corrupt list
-  for (int b = b0; b < nblocks2; b += kComThreads / 32) {
+  // the whole tree (or this workgroup's systems) is in range here: an entry
+  // or climb whose parent is not listed means a corrupt list
+  for (int b = b0; b <= blast; b += kComThreads / 32) {
     const unsigned nent = min(b == b0 ? nent0_raw : cnt2[b], (unsigned)kSeg);
     for (unsigned j = slot0; j < nent; j += 32) {
       const FrontEntry e =
           (b == b0 && j == (unsigned)slot0) ? ent0 : seg2[(size_t)b * kSeg + j];
+      const int elink = __float_as_int(e.lo.w);
+      if (part && elink >= 0 && !owns(elink & kIdMask)) continue;  // another share's
       if (staged)
         arrive_entry(e, st, index, nullptr, nullptr, error);
       else
@@ -1477,8 +1529,6 @@ __global__ __launch_bounds__(kComThreads) void com_top_kernel(
     if (threadIdx.x < kClimbWaves * 64)
       climb_ready4(nodes, box_lo, box_hi, st, (int)listed, nullptr, nullptr, error);
   }
-  __syncthreads();
-  if (threadIdx.x == 0) *top.count = 0;  // for the next build
 }
 
 // Small inputs: one workgroup sorts every key in all passes.  A onesweep
@@ -1759,7 +1809,8 @@ int com_scratch_ints(int n) {
 
 void launch_com(hipStream_t s, float4* pos_sorted, int n, Node* nodes, const int2* up,
                 unsigned* visit, float4* box_lo, float4* box_hi, int* scratch, int* error,
-                const unsigned* perm, const float4* gather_src) {
+                const unsigned* perm, const float4* gather_src, const int* sys_off,
+                int nsys) {
   if (n < 2) {  // no tree; a single particle is its own sorted position
     if (n == 1 && gather_src)
       NB_HIP(hipMemcpyAsync(pos_sorted, gather_src, sizeof(float4), hipMemcpyDeviceToDevice, s));
@@ -1777,15 +1828,17 @@ void launch_com(hipStream_t s, float4* pos_sorted, int n, Node* nodes, const int
   unsigned* rcnt1 = cnt1 + g1;
   unsigned* cnt2 = rcnt1 + g1;
   com_local_kernel<<<g1, kComThreads, 0, s>>>(pos_sorted, n, nodes, up, visit, box_lo, box_hi,
-                                              seg1, cnt1, rec1, rcnt1, perm, gather_src, error);
+                                              seg1, cnt1, rec1, rcnt1, perm, gather_src,
+                                              top.count, error);
   NB_LAUNCH_CHECK();
   if (g1 == 1) return;
   com_mid_kernel<<<g2, kComThreads, 0, s>>>(pos_sorted, n, nodes, up, box_lo, box_hi, seg1, cnt1,
                                             rec1, rcnt1, g1, seg2, cnt2, top, error);
   NB_LAUNCH_CHECK();
   if (g2 == 1) return;
-  com_top_kernel<<<1, kComThreads, 0, s>>>(pos_sorted, n, nodes, up, visit, box_lo, box_hi, seg2,
-                                           cnt2, g2, top, error);
+  const int wtop = sys_off && nsys > 1 ? std::min(nsys, kTopGroups) : 1;
+  com_top_kernel<<<wtop, kComThreads, 0, s>>>(pos_sorted, n, nodes, up, visit, box_lo, box_hi,
+                                              seg2, cnt2, g2, top, sys_off, nsys, error);
   NB_LAUNCH_CHECK();
 }
 

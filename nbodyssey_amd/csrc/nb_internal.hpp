@@ -233,10 +233,13 @@ __host__ __device__ inline bool up_root(int2 u) { return u.x == -1; }
 __host__ __device__ inline int up_parent(int2 u) { return up_root(u) ? -1 : u.x & 0x3fffffff; }
 __device__ __forceinline__ unsigned up_pass(int2 u) { return (unsigned)u.x >> 30; }
 // With gather_src, the first pass also writes pos_sorted[i] =
-// gather_src[perm[i]] (the sort then runs without its gather).
+// gather_src[perm[i]] (the sort then runs without its gather).  A forest
+// (ensemble) passes its system offsets (nsys + 1 leaf offsets on the device):
+// its systems' trees are independent, and the top pass splits over them.
 void launch_com(hipStream_t s, float4* pos_sorted, int n, Node* nodes, const int2* up,
                 unsigned* visit, float4* box_lo, float4* box_hi, int* scratch, int* error,
-                const unsigned* perm = nullptr, const float4* gather_src = nullptr);
+                const unsigned* perm = nullptr, const float4* gather_src = nullptr,
+                const int* sys_off = nullptr, int nsys = 0);
 
 // k_traverse.hip
 // Each kernel translation unit's code object is loaded by the runtime at its

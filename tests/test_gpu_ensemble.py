@@ -51,6 +51,64 @@ def local_to_forest(c, m, o, n, root):
     return np.where(c >= m - 1, (n - 1) + o + (c - (m - 1)), np.where(c == 0, root, o + c))
 
 
+def subtree_matches(t, off, n, roots, s, m, ref):
+    """System s (m particles) of forest tree t equals the standalone tree ref
+    (dict with keys, perm, pos_sorted, children, com4, size2, parent)."""
+    o = int(off[s])
+    k = t["keys"][o:o + m]
+    assert np.all((k >> np.uint64(30)) == s), s
+    assert np.array_equal(k & np.uint64((1 << 30) - 1), ref["keys"]), s
+    assert np.array_equal(t["perm"][o:o + m].astype(np.int64) - o, ref["perm"]), s
+    assert np.array_equal(t["pos_sorted"][o:o + m], ref["pos_sorted"]), s
+    g = local_to_forest(np.arange(m - 1), m, o, n, roots[s])
+    assert np.array_equal(t["children"][g], local_to_forest(ref["children"], m, o, n, roots[s])), s
+    assert np.array_equal(t["com4"][g], ref["com4"]), s
+    assert np.array_equal(t["size2"][g], ref["size2"]), s
+    # every node of the subtree but its root has its standalone parent
+    nodes = np.arange(1, 2 * m - 1)
+    assert np.array_equal(t["parent"][local_to_forest(nodes, m, o, n, roots[s])],
+                          local_to_forest(ref["parent"][1:], m, o, n, roots[s])), s
+
+
+def oracle_tree(pos):
+    r = orc.Tree(pos, key_bits=30)
+    return dict(keys=r.keys_sorted, perm=r.perm, pos_sorted=r.pos_sorted, children=r.children,
+                com4=r.com4, size2=r.size2, parent=r.parent)
+
+
+def test_forest_top_pass_shares():
+    """Forests above one 32,768-leaf superblock run the top COM pass as up to
+    16 workgroups, each owning whole systems.  24 systems of 3,000-12,000
+    particles (about 180K, six superblocks; each workgroup stages its share)
+    against the oracle's trees, then two systems of 2.4M particles, whose
+    shares each list more nodes than one workgroup stages (the unstaged
+    climb), against the standalone build of each system."""
+    rng = np.random.default_rng(17)
+    sizes = rng.integers(3000, 12001, size=24)
+    systems = [(ic.two_disk_encounter(int(m), seed=100 + k)[0], None) for k, m in enumerate(sizes)]
+    systems = [(p, np.zeros_like(p)) for p, _ in systems]
+    with engine.Ensemble(systems, eps=EPS, theta=0.5) as ens:
+        ens.compute_forces()
+        t, roots, off = ens.tree(), ens.roots(), ens.offsets
+    n = int(off[-1])
+    print(f"forest n={n} systems={len(systems)}", flush=True)
+    for s, (pos, _) in enumerate(systems):
+        subtree_matches(t, off, n, roots, s, len(pos), oracle_tree(pos))
+
+    big = [ic.two_disk_encounter(2_400_000, seed=21 + k)[0] for k in range(2)]
+    with engine.Ensemble([(p, np.zeros_like(p)) for p in big], eps=EPS, theta=0.5) as ens:
+        ens.compute_forces()
+        t, roots, off = ens.tree(), ens.roots(), ens.offsets
+    n = int(off[-1])
+    print(f"forest n={n} systems=2", flush=True)
+    for s, pos in enumerate(big):
+        with engine.Simulation(pos, eps=EPS, theta=0.5, force="bh", traverse="warp",
+                               group_width=64, key_bits=30) as sim:
+            sim.compute_forces()
+            ref = sim.tree()
+        subtree_matches(t, off, n, roots, s, len(pos), ref)
+
+
 def test_forest_matches_standalone_trees():
     systems = mixed_systems()
     with engine.Ensemble(systems, eps=EPS, theta=0.5) as ens:
