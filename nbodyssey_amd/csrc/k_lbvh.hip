@@ -48,10 +48,10 @@ namespace {
 
 template <typename K>
 __device__ __forceinline__ K make_key(float4 p, double lx, double ly, double lz, double inv) {
-  uint64_t k = (spread21(quant21(p.x, lx, inv)) << 2) | (spread21(quant21(p.y, ly, inv)) << 1) |
-               spread21(quant21(p.z, lz, inv));
-  if (sizeof(K) == 4) return (K)(k >> 33);
-  return (K)k;
+  const uint32_t qx = quant21(p.x, lx, inv), qy = quant21(p.y, ly, inv), qz = quant21(p.z, lz, inv);
+  if (sizeof(K) == 4)  // the 63-bit key's top 30 bits, interleaved in 32-bit words
+    return (K)((spread10(qx >> 11) << 2) | (spread10(qy >> 11) << 1) | spread10(qz >> 11));
+  return (K)((spread21(qx) << 2) | (spread21(qy) << 1) | spread21(qz));
 }
 
 // Digit plan of a key word: 32-bit words hold 30-bit keys, 64-bit words 63.
@@ -81,7 +81,10 @@ __device__ __forceinline__ void flush_digits(const unsigned* h, SortState* ss) {
   }
 }
 
-template <typename K>
+// PER keys per thread and pass of the grid (i, i + stride, ...): their
+// positions are loaded before the box reduction, which they do not need, and
+// all together, so the wave waits for one round trip instead of PER.
+template <typename K, int PER>
 __global__ __launch_bounds__(kBlock) void morton_hist_kernel(const float4* __restrict__ pos, int n,
                                                              const float4* __restrict__ partial,
                                                              int nparts, QuantParams* qp,
@@ -89,6 +92,11 @@ __global__ __launch_bounds__(kBlock) void morton_hist_kernel(const float4* __res
   __shared__ unsigned h[Digits<K>::kWords];
   __shared__ float s_box[kBlock / 64][6];
   __shared__ double s_q[4];
+  const int stride = gridDim.x * blockDim.x;
+  int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+  float4 p[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) p[j] = pos[min(i0 + j * stride, n - 1)];
   for (int i = threadIdx.x; i < Digits<K>::kWords; i += kBlock) h[i] = 0;
   // global bounding box from the per-workgroup partials (forces_bh.cu:158-188)
   float l[3] = {3.402823466e38f, 3.402823466e38f, 3.402823466e38f};
@@ -136,10 +144,20 @@ __global__ __launch_bounds__(kBlock) void morton_hist_kernel(const float4* __res
   }
   lds_barrier();  // s_q; the qp stores are for later launches
   const double lx = s_q[0], ly = s_q[1], lz = s_q[2], inv = s_q[3];
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    K k = make_key<K>(pos[i], lx, ly, lz, inv);
-    keys[i] = k;
-    count_digits<K>(h, k);
+  for (;;) {
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int i = i0 + j * stride;
+      if (i < n) {
+        const K k = make_key<K>(p[j], lx, ly, lz, inv);
+        keys[i] = k;
+        count_digits<K>(h, k);
+      }
+    }
+    i0 += PER * stride;
+    if (i0 >= n) break;  // grids cover n in one pass up to 2048 x 256 x PER keys
+#pragma unroll
+    for (int j = 0; j < PER; ++j) p[j] = pos[min(i0 + j * stride, n - 1)];
   }
   lds_barrier();  // the histogram; the key stores are for later launches
   flush_digits<K>(h, ss);
@@ -1708,13 +1726,23 @@ void launch_morton_hist(hipStream_t s, const float4* pos, int n, const float4* p
   // (1M keys at 16 per thread: 512 -> 256 workgroups, 18.1 -> 16.2 us; 128
   // was slower); small systems keep 8 per thread for parallelism (at 2K-64K
   // keys 16 per thread took 10.9 us on average instead of 7.9).
-  int g = grid_for(n, n >= (1 << 19) ? 16 : 8, 2048);
-  if (key_bits == 30)
-    morton_hist_kernel<uint32_t><<<g, kBlock, 0, s>>>(pos, n, partial, nparts, qp,
-                                                      (uint32_t*)keys, ss);
-  else
-    morton_hist_kernel<uint64_t><<<g, kBlock, 0, s>>>(pos, n, partial, nparts, qp,
-                                                      (uint64_t*)keys, ss);
+  const bool big = n >= (1 << 19);
+  const int g = grid_for(n, big ? 16 : 8, 2048);
+  if (key_bits == 30) {
+    if (big)
+      morton_hist_kernel<uint32_t, 16><<<g, kBlock, 0, s>>>(pos, n, partial, nparts, qp,
+                                                            (uint32_t*)keys, ss);
+    else
+      morton_hist_kernel<uint32_t, 8><<<g, kBlock, 0, s>>>(pos, n, partial, nparts, qp,
+                                                           (uint32_t*)keys, ss);
+  } else {
+    if (big)
+      morton_hist_kernel<uint64_t, 16><<<g, kBlock, 0, s>>>(pos, n, partial, nparts, qp,
+                                                            (uint64_t*)keys, ss);
+    else
+      morton_hist_kernel<uint64_t, 8><<<g, kBlock, 0, s>>>(pos, n, partial, nparts, qp,
+                                                           (uint64_t*)keys, ss);
+  }
   NB_LAUNCH_CHECK();
 }
 

@@ -25,6 +25,18 @@ __device__ __forceinline__ uint64_t spread21(uint64_t v) {
   return v;
 }
 
+// spread10 interleaves v's low 10 bits with two zero bits each (32-bit ops).
+// For a 21-bit q, spread10(q >> 11) is spread21(q) >> 33 on every third
+// bit: the 30-bit key is the top 30 bits of the 63-bit key.
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v | v << 16) & 0x030000ffu;
+  v = (v | v << 8) & 0x0300f00fu;
+  v = (v | v << 4) & 0x030c30c3u;
+  v = (v | v << 2) & 0x09249249u;
+  return v;
+}
+
 __device__ __forceinline__ uint32_t quant21(float p, double lo, double inv) {
   double v = ((double)p - lo) * inv * 2097152.0;
   v = (v > 0.0) ? v : 0.0;
