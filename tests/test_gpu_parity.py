@@ -52,6 +52,18 @@ def test_morton_bit_exact(case, bits):
     assert np.array_equal(engine.morton_keys(pos, bits), orc.morton(pos, bits))
 
 
+def test_morton_bit_exact_multi_pass():
+    """Above 2048 workgroups x 256 threads x 16 keys, each thread of the Morton
+    pass takes its keys in more than one pass over the grid: 9,000,001 keys
+    are two passes, the second partial."""
+    rng = np.random.default_rng(5)
+    p = rng.normal(0.0, 3.0, size=(9_000_001, 3)).astype(np.float32)
+    pos = np.column_stack([p, np.ones(len(p), np.float32)])
+    for bits in (63, 30):
+        print(f"morton multi-pass n={len(pos)} bits={bits}", flush=True)
+        assert np.array_equal(engine.morton_keys(pos, bits), orc.morton(pos, bits)), bits
+
+
 @pytest.mark.parametrize("bits", [63, 30])
 @pytest.mark.parametrize("n", [1, 2, 100, 4095, 4096, 4097, 65537, 300001])
 def test_sort_bit_exact(n, bits):
