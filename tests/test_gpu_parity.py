@@ -197,7 +197,8 @@ def test_theta0_matches_allpairs_65536(walk):
 
 def test_allpairs_vs_oracle_12k():
     """Naive kernel vs the oracle's FP32 j-order restatement of force_kernel
-    (median <= 1e-5 relative) and vs FP64."""
+    (median <= 1e-5 relative: the accelerations must match the naive kernel
+    within 1e-5) and vs FP64."""
     pos, _ = ic.two_disk_encounter(12000, seed=0, eps=0.05)
     with engine.Simulation(pos, eps=0.05, force="naive") as sim:
         a = sim.compute_forces()
