@@ -558,35 +558,100 @@ __global__ __launch_bounds__(kBlock) void karras_kernel(const K* __restrict__ ke
   }
   __syncthreads();
   const KeyWindow<K> key{keys, (const lds<K>*)s_key, wlo};
+  const int lane = threadIdx.x & 63;
   const int i = base + threadIdx.x;
-  if (i >= n - 1) return;
-  const K ki = key(i);
-  int d = (delta_w(key, n, i, ki, i + 1) - delta_w(key, n, i, ki, i - 1)) >= 0 ? 1 : -1;
-  int dmin = delta_w(key, n, i, ki, i - d);
-  int lmax = 2;
-  while (delta_w(key, n, i, ki, i + lmax * d) > dmin) lmax <<= 1;
-  int l = 0;
-  for (int t = lmax >> 1; t > 0; t >>= 1)
-    if (delta_w(key, n, i, ki, i + (l + t) * d) > dmin) l += t;
-  int j = i + l * d;
-  int first = min(i, j), last = max(i, j);
-  const K kf = key(first);
-  int common = delta_w(key, n, first, kf, last);
-  int split = first, step = last - first;
-  do {
-    step = (step + 1) >> 1;
-    if (split + step < last && delta_w(key, n, first, kf, split + step) > common) split += step;
-  } while (step > 1);
-  int left = (split == first) ? (n - 1 + split) : split;
-  int right = (split + 1 == last) ? (n - 1 + split + 1) : split + 1;
-  *reinterpret_cast<int2*>(&nodes[i].left) = make_int2(left, right);
-  nodes[i].other = j;
-  // each child's upward record: parent with the parent's pass word, sibling
-  const unsigned vw = com_visit_word(first, last);
-  up[left] = make_up(i, vw, right);
-  up[right] = make_up(i, vw, left);
-  visit[i] = vw;
-  if (i == 0) up[0] = up_none();
+  const bool active = i < n - 1;
+  // node i's record, children's upward records and pass word
+  auto emit = [&](int node, int j, int first, int last, int split) {
+    const int left = (split == first) ? (n - 1 + split) : split;
+    const int right = (split + 1 == last) ? (n - 1 + split + 1) : split + 1;
+    *reinterpret_cast<int2*>(&nodes[node].left) = make_int2(left, right);
+    nodes[node].other = j;
+    // each child's upward record: parent with the parent's pass word, sibling
+    const unsigned vw = com_visit_word(first, last);
+    up[left] = make_up(node, vw, right);
+    up[right] = make_up(node, vw, left);
+    visit[node] = vw;
+    if (node == 0) up[0] = up_none();
+  };
+  // Each lane searches its node's range while it stays within kKarrasWin of
+  // i (probes in the LDS window).  A longer range would make the whole wave
+  // wait for that lane's chain of global probes (three searches of log2 of
+  // the range each), so such nodes are marked and resolved afterwards by the
+  // whole wave, 64 probes per round trip (below).
+  int d = 1, dmin = 0;
+  K ki = 0;
+  bool is_long = false;
+  if (active) {
+    ki = key(i);
+    d = (delta_w(key, n, i, ki, i + 1) - delta_w(key, n, i, ki, i - 1)) >= 0 ? 1 : -1;
+    dmin = delta_w(key, n, i, ki, i - d);
+    int lmax = 2;
+    while (lmax <= kKarrasWin && delta_w(key, n, i, ki, i + lmax * d) > dmin) lmax <<= 1;
+    if (lmax > kKarrasWin) {
+      is_long = true;
+    } else {
+      int l = 0;
+      for (int t = lmax >> 1; t > 0; t >>= 1)
+        if (delta_w(key, n, i, ki, i + (l + t) * d) > dmin) l += t;
+      const int j = i + l * d;
+      const int first = min(i, j), last = max(i, j);
+      const K kf = key(first);
+      const int common = delta_w(key, n, first, kf, last);
+      int split = first, step = last - first;
+      do {
+        step = (step + 1) >> 1;
+        if (split + step < last && delta_w(key, n, first, kf, split + step) > common)
+          split += step;
+      } while (step > 1);
+      emit(i, j, first, last, split);
+    }
+  }
+  // Long ranges, one node at a time for the wave.  Both searches look for
+  // the last position of a monotone predicate between a known true lo and a
+  // known false hi: lane k probes lo + (k + 1) * step, and the count of true
+  // probes narrows [lo, hi) 64-fold per round trip.  The results are those
+  // of the doubling and bisection above (same predicate, same bounds).
+  auto last_true = [&](int lo, int hi, int from, K kfrom, int dir, int thresh) {
+    while (hi - lo > 1) {
+      const int step = (hi - lo + 63) / 64;
+      const int off = lo + (lane + 1) * step;
+      const bool tv = off < hi && delta_w(key, n, from, kfrom, from + off * dir) > thresh;
+      lo += __popcll(__builtin_amdgcn_ballot_w64(tv)) * step;
+      hi = min(hi, lo + step);
+    }
+    return lo;
+  };
+  for (unsigned long long lm = __builtin_amdgcn_ballot_w64(is_long); lm; lm &= lm - 1) {
+    const int src = __builtin_ctzll(lm);
+    const int li = __builtin_amdgcn_readlane(i, src);
+    const int ld = __builtin_amdgcn_readlane(d, src);
+    const int lmin = __builtin_amdgcn_readlane(dmin, src);
+    K lk;
+    if (sizeof(K) == 4) {
+      lk = (K)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)ki, src);
+    } else {
+      const uint64_t k64 = (uint64_t)ki;
+      lk = (K)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(k64 >> 32), src)
+                << 32) |
+               (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)k64, src));
+    }
+    // range end: delta(li + l d) > lmin holds at l = kKarrasWin; bracket it
+    // between kKarrasWin << (m - 1) and kKarrasWin << m (lane k probes
+    // kKarrasWin << k), then narrow
+    const long long far = (long long)kKarrasWin << lane;
+    const bool bt = lane < 24 && far <= (long long)n &&
+                    delta_w(key, n, li, lk, li + (int)far * ld) > lmin;
+    const int m = __popcll(__builtin_amdgcn_ballot_w64(bt));
+    const int l = last_true(kKarrasWin << (m - 1), (int)min((long long)n, (long long)kKarrasWin << m),
+                            li, lk, ld, lmin);
+    const int j = li + l * ld;
+    const int first = min(li, j), last = max(li, j);
+    const K kf = key(first);
+    const int common = delta_w(key, n, first, kf, last);
+    const int split = first + last_true(0, last - first, first, kf, 1, common);
+    if (lane == src) emit(li, j, first, last, split);
+  }
 }
 
 // Node data a climbing thread carries: centre of mass (x, y, z, m) and AABB.
